@@ -1,0 +1,230 @@
+"""Benchmark of the MI355X force path on the single-system workload.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--variant NAME] [--no-cpu-baseline]
+
+Workload: one system of N = 98304 bodies, positions uniform in a ball of
+radius 1.54 (N/16384)^(1/3), unit masses, velocities uniform in a unit ball
+(seeded), softeningSquared = 0.01, dt = 0.016, damping 0.95 (src/nbody.cu:152).
+A step is one force evaluation over all N^2 pairs plus the integrator, with
+the state resident in HBM. The timed region is K steps between device
+synchronisations; the line printed is
+
+    value = N^2 K / time / 1e9   (10^9 interactions/s, the reference's figure,
+                                  src/nbody.cu:1026-1027)
+
+plus the roofline of the force kernel (20 flops per interaction against the
+FP32 vector peak; the kernel time comes from events the library records
+around the force kernel itself, on its stream, over the timed region), the
+accuracy of the last force against the FP64 direct sum (the library's GPU
+diagnostic), and a CPU baseline: the reference's CPU_SOA_tiled
+(src/nbody_CPU_SOA_tiled.cpp:51-115) compiled from its sources when that
+build is present, else the oracle's restatement of it, timed on this host's
+cores on a bounded number of the same steps.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector / dense matrix peak (published)
+PEAK_FP32_NONPACKED = 78.65    # the same without packed-FP32 instructions (half rate)
+REF_BEST_GPU = 501.5           # 10^9 int/s, GPU_Shared clang build, RTX 2080 Ti, N=98304 (README.md:230-233)
+METRIC = "10^9 interactions/s + % FP32 peak at N=96K; ensemble throughput + p99 latency"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=98304)
+    ap.add_argument("--variant", default="auto", choices=["auto", "sgpr", "lds", "dpp", "aos"])
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget-s", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s):
+    """Time CPU_SOA_tiled + integrate on this host (rank 0, N = 1 only)."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import numpy as np
+    from oracle_lib import oracle, ref_cpu
+    n = pm_host.shape[0]
+    ref = ref_cpu()
+    kind, impl = ("reference", ref) if ref is not None else ("port", oracle())
+    integ = oracle()
+    pm, vm = pm_host.copy(), vm_host.copy()
+    zero = np.zeros((n, 4), np.float32)
+    times = []
+    # the reference's CPU_SOA_tiled adds into a force zeroed by the caller (src/nbody.cu:328-331)
+    f, ms = impl.cpu_soa_tiled(pm, eps2) if n % 4096 == 0 else impl.cpu_soa(pm, eps2)  # warm-up
+    spent = 0.0
+    steps = 0
+    while steps < 20:
+        t0 = time.perf_counter()
+        f, ms = impl.cpu_soa_tiled(pm, eps2) if n % 4096 == 0 else impl.cpu_soa(pm, eps2)
+        pm, vm = integ.integrate_aos(pm, vm, f if f is not None else zero, dt, damping)
+        times.append(ms / 1e3)
+        spent += time.perf_counter() - t0
+        steps += 1
+        if steps >= 3 and spent + spent / steps > budget_s:
+            break
+    med = float(np.median(times))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    algo = "CPU_SOA_tiled" if n % 4096 == 0 else "CPU_SOA"
+    return {
+        "value": round(n * n / med / 1e9, 3),
+        "unit": "10^9 interactions/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"{steps} of the workload's steps ({algo} force, median of its own steady_clock ms, "
+                  f"+ host integrate), N={n}, OMP_NUM_THREADS={threads}, OMP_PROC_BIND="
+                  f"{os.environ.get('OMP_PROC_BIND', '')}",
+        "ms_per_force": round(med * 1e3, 2),
+        "cpu": cpu_model(),
+    }
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def traffic_from_profiles(workload_key):
+    """HBM bytes per force launch measured by rocprofv3 counters (FETCH_SIZE,
+    WRITE_SIZE, calibrated on a copy kernel), committed under profiles/."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        data = json.load(open(path))
+        return data[workload_key]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path. "
+              "The single-system work would shard by i-range (nbody_hip_force base/n) with an all-gather "
+              "of positions per step.", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+
+    import numpy as np
+    import torch
+    sys.path.insert(0, HERE)
+    from nbody_amd import _native, ics
+
+    L = _native.lib()
+    n, K, W = args.n, args.steps, args.warmup
+    eps2, dt, damping = 0.01, 0.016, 0.95
+    pm_h, vm_h = ics.uniform_ball(n, seed=1)
+    dev = torch.device("cuda:0")
+    pm = torch.from_numpy(pm_h).to(dev)
+    vm = torch.from_numpy(vm_h).to(dev)
+    force = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    variant = {"auto": _native.AUTO, "sgpr": _native.SGPR, "lds": _native.LDS, "dpp": _native.DPP,
+               "aos": _native.AOS}[args.variant]
+    opt = _native.Options.make(variant=variant)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+
+    if not args.no_autotune and variant != _native.AOS:
+        _native.check_ms(L.nbody_hip_autotune(P(force), P(pm), eps2, n, variant), "autotune")
+    plan = _native.Options()
+    _native.check(L.nbody_hip_plan(n, n, ctypes.byref(opt), ctypes.byref(plan)), "plan")
+
+    def step():
+        _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, 0, n, ctypes.byref(opt), stream), "force")
+        _native.check(L.nbody_hip_integrate(P(pm), P(vm), P(force), dt, damping, n, stream), "integrate")
+
+    for _ in range(W):
+        step()
+    torch.cuda.synchronize()
+    _native.check(L.nbody_hip_kernel_timer(1), "timer")
+    t0 = time.perf_counter()
+    for _ in range(K):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kms = ctypes.c_double()
+    launches = ctypes.c_int()
+    _native.check(L.nbody_hip_kernel_timer_read(ctypes.byref(kms), ctypes.byref(launches)), "timer read")
+    L.nbody_hip_kernel_timer(0)
+
+    # accuracy of the last force (state before the last integrate is gone; recompute on the final state)
+    step_force = torch.zeros_like(force)
+    _native.check(L.nbody_hip_force(P(step_force), P(pm), eps2, n, 0, n, ctypes.byref(opt), stream), "force")
+    f64 = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    _native.check_ms(L.nbody_hip_force_fp64(P(f64), P(pm), eps2, n), "fp64")
+    torch.cuda.synchronize()
+    d = (step_force[:, :3].double() - f64[:, :3]).norm(dim=1) / f64[:, :3].norm(dim=1)
+    max_rel, rms_rel = float(d.max()), float(d.pow(2).mean().sqrt())
+
+    inter = float(n) * n
+    value = inter * K / elapsed / 1e9
+    kernel_ms = kms.value / max(launches.value, 1)
+    achieved = 20.0 * inter / (kernel_ms * 1e-3) / 1e12
+    cus = ctypes.c_int()
+    clk = ctypes.c_int()
+    arch = ctypes.create_string_buffer(64)
+    L.nbody_hip_device_info(ctypes.byref(cus), ctypes.byref(clk), arch, 64)
+    workload = f"uniform_ball_n{n}"
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "10^9 interactions/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(elapsed * 1e3 / K, 4),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": round(value / REF_BEST_GPU, 2),
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform ball, unit masses)",
+        "config": {
+            "workload": f"single system, N={n} uniform ball radius {ics.ball_radius(n):.4f}, unit masses, "
+                        f"eps^2={eps2}, dt={dt}, damping={damping}; step = force + integrate, state in HBM",
+            "n_bodies": n,
+            "interactions_per_step": int(inter),
+            "plan": {"variant": _native.VARIANT_NAMES[plan.variant], "i_per_lane": plan.i_per_lane,
+                     "jslice": plan.jslice, "compensated": bool(plan.compensated), "newton": bool(plan.newton)},
+            "vs_baseline_ref": "GPU_Shared, RTX 2080 Ti, clang build: 501.5 x10^9 int/s (README.md:230-233)",
+            "device": f"{arch.value.decode()} {cus.value} CUs {clk.value // 1000} MHz",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 2),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": traffic_from_profiles(workload),
+            "kernel_ms": round(kernel_ms, 4),
+            "kernel": f"k_force_{_native.VARIANT_NAMES[plan.variant].lower()}",
+            "flops_per_launch": 20 * int(inter),
+            "peak_without_packed_fp32": PEAK_FP32_NONPACKED,
+            "frac_without_packed_fp32": round(achieved / PEAK_FP32_NONPACKED, 4),
+        },
+        "accuracy": {"max_rel": max_rel, "rms_rel": rms_rel,
+                     "vs": "FP64 direct sum (GPU diagnostic), per-body |F-F64|/|F64|"},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pm_h, vm_h, np.float32(eps2), dt, damping, args.cpu_budget_s)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

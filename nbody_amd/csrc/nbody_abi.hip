@@ -1,0 +1,568 @@
+// extern "C" entry points of libnbody_hip.so (declared in include/nbody_hip.h).
+//
+// Error convention of the reference (src/chError.h:86-91 + src/nbody.cu:385-386):
+// a failing table entry returns 0.0f and the driver skips it. Here the reason is
+// also kept per thread for nbody_hip_last_error().
+#include "nbody_hip.h"
+#include "nbody_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace nbody;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+enum : int { OK = 0, E_ARG = 1, E_HIP = 2, E_UNSUPPORTED = 3 };
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                      __LINE__);                                                       \
+            return E_HIP;                                                              \
+        }                                                                              \
+    } while (0)
+
+// Largest N accepted: float4 offsets and 4*N fit comfortably in int.
+constexpr size_t kMaxBodies = size_t(1) << 28;
+
+// Device buffer that only grows. Growing waits for the device first, so a
+// buffer still read by queued kernels is never freed under them.
+struct Buffer {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) return e;
+            (void)hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max(bytes, cap + cap / 2);
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+};
+
+struct DeviceState {
+    bool init = false;
+    int cus = 0;
+    int clock_khz = 0;
+    char arch[64] = {0};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Buffer partial;   // FP32 slice partials
+    Buffer partial64; // FP64 slice partials / energy partials
+    Buffer stage_pm, stage_force, small;
+    // tuned plans: (variant, N, n, comp, newton) -> plan
+    std::map<std::tuple<int, size_t, size_t, int, int>, ForcePlan> tuned;
+    // kernel timer: event pairs recorded around each force kernel while enabled
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timer_events;
+    size_t timer_used = 0;
+};
+
+// Largest number of force launches the kernel timer records between reads.
+constexpr size_t kMaxTimedLaunches = 1 << 14;
+
+std::mutex g_mutex;
+std::vector<DeviceState> g_devices;
+
+int device_state(DeviceState** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if ((int)g_devices.size() <= dev) g_devices.resize(dev + 1);
+    DeviceState& d = g_devices[dev];
+    if (!d.init) {
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        d.cus = prop.multiProcessorCount;
+        d.clock_khz = prop.clockRate;
+        snprintf(d.arch, sizeof d.arch, "%s", prop.gcnArchName);
+        HIP_TRY(hipEventCreate(&d.ev0));
+        HIP_TRY(hipEventCreate(&d.ev1));
+        d.init = true;
+    }
+    *out = &d;
+    return OK;
+}
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+// Default decomposition: enough blocks for ~48 waves per SIMD (the dispatcher
+// balances them across the 256 CUs), j slices of at least 1024 bodies in
+// multiples of 256.
+ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip_options* opt)
+{
+    ForcePlan p;
+    p.variant = variant;
+    p.newton = opt && opt->newton > 0;
+    const int comp_opt = opt ? opt->compensated : -1;
+    p.comp = comp_opt < 0 ? (N > 65536) : comp_opt > 0;
+    if (variant == VAR_AOS) {
+        p.ipl = 1;
+        p.jslice = (int)N;
+        p.nslices = 1;
+        p.comp = false;
+        return p;
+    }
+    p.ipl = (opt && opt->i_per_lane > 0) ? opt->i_per_lane : 1;
+    if (opt && opt->jslice > 0) {
+        p.jslice = opt->jslice;
+    } else {
+        const long blocks_i = (long)((n + 256 * p.ipl - 1) / (256 * p.ipl));
+        const long target = (long)std::max(cus, 1) * 48;
+        long ns = std::max(1L, (target + blocks_i - 1) / blocks_i);
+        const long max_ns = std::max(1L, (long)(N / 1024));
+        ns = std::min(ns, max_ns);
+        p.jslice = round_up((int)((N + ns - 1) / ns), 256);
+    }
+    p.jslice = std::max(1, std::min(p.jslice, (int)N));
+    p.nslices = (int)((N + p.jslice - 1) / p.jslice);
+    return p;
+}
+
+int resolve_variant(const nbody_hip_options* opt)
+{
+    int v = opt ? opt->variant : NBODY_HIP_AUTO;
+    if (v == NBODY_HIP_AUTO) v = VAR_SGPR;
+    return v;
+}
+
+int make_plan(DeviceState& d, size_t N, size_t n, const nbody_hip_options* opt, ForcePlan* plan)
+{
+    const int variant = resolve_variant(opt);
+    if (variant < 0 || variant >= VAR_COUNT) {
+        set_error("unknown variant %d", variant);
+        return E_ARG;
+    }
+    if (variant == VAR_SYMMETRIC) {
+        set_error("the symmetric variant is not available in this build");
+        return E_UNSUPPORTED;
+    }
+    if (opt && opt->i_per_lane != 0 && opt->i_per_lane != 1 && opt->i_per_lane != 2 && opt->i_per_lane != 4) {
+        set_error("i_per_lane must be 1, 2 or 4 (got %d)", opt->i_per_lane);
+        return E_ARG;
+    }
+    ForcePlan p = default_plan(variant, N, n, d.cus, opt);
+    const bool explicit_shape = opt && (opt->i_per_lane > 0 || opt->jslice > 0);
+    if (!explicit_shape) {
+        auto it = d.tuned.find(std::make_tuple(variant, N, n, (int)p.comp, (int)p.newton));
+        if (it != d.tuned.end()) p = it->second;
+    }
+    *plan = p;
+    return OK;
+}
+
+int check_args(const void* force, const void* pm, size_t N, size_t base, size_t n)
+{
+    if (!force || !pm) {
+        set_error("null buffer");
+        return E_ARG;
+    }
+    if (N < 1 || N > kMaxBodies) {
+        set_error("N = %zu outside [1, %zu]", N, kMaxBodies);
+        return E_ARG;
+    }
+    if (base > N || n > N - base) {
+        set_error("i range [%zu, %zu) outside [0, %zu)", base, base + n, N);
+        return E_ARG;
+    }
+    return OK;
+}
+
+// Queue one force evaluation on `stream` (caller holds g_mutex).
+int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
+              size_t n, hipStream_t stream)
+{
+    ForceParams p;
+    p.pm = reinterpret_cast<const float4*>(pm);
+    p.n = (int)N;
+    p.i0 = (int)base;
+    p.ni = (int)n;
+    p.eps2 = eps2;
+    p.jslice = plan.jslice;
+    p.nslices = plan.nslices;
+    p.force = force;
+    const size_t bytes = plan_scratch_bytes(plan, (int)n);
+    if (bytes) {
+        HIP_TRY(d.partial.reserve(bytes));
+        p.hi = static_cast<float4*>(d.partial.ptr);
+        p.lo = plan.comp ? p.hi + (size_t)plan.nslices * n : nullptr;
+    }
+    hipEvent_t before = nullptr, after = nullptr;
+    if (d.timing && d.timer_used < kMaxTimedLaunches) {
+        if (d.timer_used == d.timer_events.size()) {
+            hipEvent_t a, b;
+            HIP_TRY(hipEventCreate(&a));
+            HIP_TRY(hipEventCreate(&b));
+            d.timer_events.emplace_back(a, b);
+        }
+        before = d.timer_events[d.timer_used].first;
+        after = d.timer_events[d.timer_used].second;
+        d.timer_used++;
+    }
+    HIP_TRY(launch_force(plan, p, stream, before, after));
+    return OK;
+}
+
+// Synchronous, event-timed evaluation on the null stream; returns ms or 0.
+float timed_force(const nbody_hip_options* opt, float* force, const float* pm, float eps2, size_t N)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (check_args(force, pm, N, 0, N)) return 0.0f;
+    DeviceState* d;
+    if (device_state(&d)) return 0.0f;
+    ForcePlan plan;
+    if (make_plan(*d, N, N, opt, &plan)) return 0.0f;
+    // scratch is reserved outside the timed region
+    const size_t bytes = plan_scratch_bytes(plan, (int)N);
+    if (bytes && d->partial.reserve(bytes) != hipSuccess) {
+        set_error("cannot allocate %zu bytes of slice scratch", bytes);
+        return 0.0f;
+    }
+    if (hipEventRecord(d->ev0, nullptr) != hipSuccess) return 0.0f;
+    if (run_force(*d, plan, force, pm, eps2, N, 0, N, nullptr)) return 0.0f;
+    float ms = 0.0f;
+    hipError_t e = hipEventRecord(d->ev1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(d->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, d->ev0, d->ev1);
+    if (e != hipSuccess) {
+        set_error("timing failed: %s", hipGetErrorString(e));
+        return 0.0f;
+    }
+    // a kernel that ran in no time at all would read as "skipped" to the caller
+    return std::max(ms, 1e-6f);
+}
+
+float variant_entry(int variant, float* force, float* pm, float eps2, size_t N)
+{
+    nbody_hip_options o{variant, 0, 0, -1, 0};
+    return timed_force(&o, force, pm, eps2, N);
+}
+
+}  // namespace
+
+extern "C" {
+
+float ComputeGravitation_HIP_AOS(float* force, float* posMass, float softeningSquared, size_t N)
+{
+    return variant_entry(NBODY_HIP_AOS, force, posMass, softeningSquared, N);
+}
+
+float ComputeGravitation_HIP_LDS(float* force, float* posMass, float softeningSquared, size_t N)
+{
+    return variant_entry(NBODY_HIP_LDS, force, posMass, softeningSquared, N);
+}
+
+float ComputeGravitation_HIP_DPP(float* force, float* posMass, float softeningSquared, size_t N)
+{
+    return variant_entry(NBODY_HIP_DPP, force, posMass, softeningSquared, N);
+}
+
+float ComputeGravitation_HIP_SGPR(float* force, float* posMass, float softeningSquared, size_t N)
+{
+    return variant_entry(NBODY_HIP_SGPR, force, posMass, softeningSquared, N);
+}
+
+float ComputeGravitation_HIP_Symmetric(float* force, float* posMass, float softeningSquared, size_t N)
+{
+    return variant_entry(NBODY_HIP_SYMMETRIC, force, posMass, softeningSquared, N);
+}
+
+float ComputeGravitation_HIP(float* force, float* posMass, float softeningSquared, size_t N)
+{
+    return variant_entry(NBODY_HIP_AUTO, force, posMass, softeningSquared, N);
+}
+
+float ComputeGravitation_HIP_Host(float* force, float* posMass, float softeningSquared, size_t N)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (check_args(force, posMass, N, 0, N)) return 0.0f;
+    DeviceState* d;
+    if (device_state(&d)) return 0.0f;
+    ForcePlan plan;
+    if (make_plan(*d, N, N, nullptr, &plan)) return 0.0f;
+    const size_t bytes = N * 4 * sizeof(float);
+    const size_t sbytes = plan_scratch_bytes(plan, (int)N);
+    if (d->stage_pm.reserve(bytes) != hipSuccess || d->stage_force.reserve(bytes) != hipSuccess ||
+        (sbytes && d->partial.reserve(sbytes) != hipSuccess)) {
+        set_error("cannot allocate staging buffers for N = %zu", N);
+        return 0.0f;
+    }
+    float* dpm = static_cast<float*>(d->stage_pm.ptr);
+    float* dforce = static_cast<float*>(d->stage_force.ptr);
+    float ms = 0.0f;
+    hipError_t e = hipEventRecord(d->ev0, nullptr);
+    if (e == hipSuccess) e = hipMemcpyAsync(dpm, posMass, bytes, hipMemcpyHostToDevice, nullptr);
+    // the caller's force w components travel both ways untouched
+    if (e == hipSuccess) e = hipMemcpyAsync(dforce, force, bytes, hipMemcpyHostToDevice, nullptr);
+    if (e != hipSuccess) {
+        set_error("host-to-device copy failed: %s", hipGetErrorString(e));
+        return 0.0f;
+    }
+    if (run_force(*d, plan, dforce, dpm, softeningSquared, N, 0, N, nullptr)) return 0.0f;
+    e = hipMemcpyAsync(force, dforce, bytes, hipMemcpyDeviceToHost, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(d->ev1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(d->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, d->ev0, d->ev1);
+    if (e != hipSuccess) {
+        set_error("device-to-host copy or timing failed: %s", hipGetErrorString(e));
+        return 0.0f;
+    }
+    return std::max(ms, 1e-6f);
+}
+
+int nbody_hip_force(float* force, const float* posMass, float softeningSquared, size_t N, size_t base, size_t n,
+                    const nbody_hip_options* opt, void* stream)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (int r = check_args(force, posMass, N, base, n)) return r;
+    if (n == 0) return OK;
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    ForcePlan plan;
+    if (int r = make_plan(*d, N, n, opt, &plan)) return r;
+    return run_force(*d, plan, force, posMass, softeningSquared, N, base, n, static_cast<hipStream_t>(stream));
+}
+
+int nbody_hip_integrate(float* posMass, float* velInvMass, const float* force, float dt, float damping, size_t N,
+                        void* stream)
+{
+    if (!posMass || !velInvMass || !force) {
+        set_error("null buffer");
+        return E_ARG;
+    }
+    if (N > kMaxBodies) {
+        set_error("N = %zu too large", N);
+        return E_ARG;
+    }
+    HIP_TRY(launch_integrate(reinterpret_cast<float4*>(posMass), reinterpret_cast<float4*>(velInvMass),
+                             reinterpret_cast<const float4*>(force), dt, damping, (int)N,
+                             static_cast<hipStream_t>(stream)));
+    return OK;
+}
+
+float integrateGravitation_HIP(float* posMass, float* velInvMass, const float* force, float dt, float damping,
+                               size_t N)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* d;
+    if (device_state(&d)) return 0.0f;
+    if (hipEventRecord(d->ev0, nullptr) != hipSuccess) return 0.0f;
+    if (nbody_hip_integrate(posMass, velInvMass, force, dt, damping, N, nullptr)) return 0.0f;
+    float ms = 0.0f;
+    hipError_t e = hipEventRecord(d->ev1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(d->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, d->ev0, d->ev1);
+    if (e != hipSuccess) {
+        set_error("integrate failed: %s", hipGetErrorString(e));
+        return 0.0f;
+    }
+    return std::max(ms, 1e-6f);
+}
+
+float nbody_hip_force_fp64(double* force64, const float* posMass, double softeningSquared, size_t N)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (check_args(force64, posMass, N, 0, N)) return 0.0f;
+    DeviceState* d;
+    if (device_state(&d)) return 0.0f;
+    const int jslice = 2048;
+    const size_t nslices = (N + jslice - 1) / jslice;
+    if (d->partial64.reserve(nslices * N * 3 * sizeof(double)) != hipSuccess) {
+        set_error("cannot allocate FP64 scratch");
+        return 0.0f;
+    }
+    float ms = 0.0f;
+    hipError_t e = hipEventRecord(d->ev0, nullptr);
+    if (e == hipSuccess)
+        e = launch_force_fp64(force64, reinterpret_cast<const float4*>(posMass), (int)N, 0, (int)N, softeningSquared,
+                              jslice, static_cast<double*>(d->partial64.ptr), nullptr);
+    if (e == hipSuccess) e = hipEventRecord(d->ev1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(d->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, d->ev0, d->ev1);
+    if (e != hipSuccess) {
+        set_error("FP64 force failed: %s", hipGetErrorString(e));
+        return 0.0f;
+    }
+    return std::max(ms, 1e-6f);
+}
+
+int nbody_hip_energy_fp64(double* out, const float* posMass, const float* velInvMass, double softeningSquared,
+                          size_t N)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!out || !velInvMass) {
+        set_error("null buffer");
+        return E_ARG;
+    }
+    if (int r = check_args(posMass, posMass, N, 0, N)) return r;
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    const int blocks = fp64_energy_blocks((int)N);
+    HIP_TRY(d->partial64.reserve((size_t)blocks * 5 * sizeof(double)));
+    HIP_TRY(d->small.reserve(8 * sizeof(double)));
+    double* acc = static_cast<double*>(d->small.ptr);
+    HIP_TRY(launch_energy_fp64(acc, reinterpret_cast<const float4*>(posMass),
+                               reinterpret_cast<const float4*>(velInvMass), (int)N, softeningSquared,
+                               static_cast<double*>(d->partial64.ptr), nullptr));
+    HIP_TRY(hipMemcpy(out, acc, 5 * sizeof(double), hipMemcpyDeviceToHost));
+    return OK;
+}
+
+int nbody_hip_plan(size_t N, size_t n, const nbody_hip_options* opt, nbody_hip_options* out)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!out) {
+        set_error("null plan");
+        return E_ARG;
+    }
+    if (N < 1 || N > kMaxBodies || n > N) {
+        set_error("bad sizes N = %zu, n = %zu", N, n);
+        return E_ARG;
+    }
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    ForcePlan p;
+    if (int r = make_plan(*d, N, n, opt, &p)) return r;
+    out->variant = p.variant;
+    out->i_per_lane = p.ipl;
+    out->jslice = p.jslice;
+    out->compensated = p.comp ? 1 : 0;
+    out->newton = p.newton ? 1 : 0;
+    return OK;
+}
+
+float nbody_hip_autotune(float* force, const float* posMass, float softeningSquared, size_t N, int variant)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (check_args(force, posMass, N, 0, N)) return 0.0f;
+    DeviceState* d;
+    if (device_state(&d)) return 0.0f;
+    nbody_hip_options base{variant, 0, 0, -1, 0};
+    ForcePlan p0;
+    if (make_plan(*d, N, N, &base, &p0)) return 0.0f;
+    if (variant == NBODY_HIP_AUTO) variant = p0.variant;
+    std::vector<ForcePlan> cands;
+    if (p0.variant == VAR_AOS) {
+        cands.push_back(p0);
+    } else {
+        for (int ipl : {1, 2, 4}) {
+            nbody_hip_options o = base;
+            o.variant = p0.variant;
+            o.i_per_lane = ipl;
+            ForcePlan q = default_plan(p0.variant, N, N, d->cus, &o);
+            for (int mul : {1, 2, 4}) {
+                for (int div : {1, 2, 4}) {
+                    if (mul > 1 && div > 1) continue;
+                    ForcePlan c = q;
+                    c.jslice = std::min((int)N, round_up(std::max(256, q.jslice * mul / div), 256));
+                    c.nslices = (int)((N + c.jslice - 1) / c.jslice);
+                    cands.push_back(c);
+                }
+            }
+        }
+    }
+    float best = 0.0f;
+    ForcePlan best_plan = p0;
+    for (const ForcePlan& c : cands) {
+        const size_t bytes = plan_scratch_bytes(c, (int)N);
+        if (bytes && d->partial.reserve(bytes) != hipSuccess) continue;
+        float t = 0.0f;
+        for (int rep = 0; rep < 3; ++rep) {
+            if (hipEventRecord(d->ev0, nullptr) != hipSuccess) return 0.0f;
+            if (run_force(*d, c, force, posMass, softeningSquared, N, 0, N, nullptr)) return 0.0f;
+            float ms = 0.0f;
+            if (hipEventRecord(d->ev1, nullptr) != hipSuccess || hipEventSynchronize(d->ev1) != hipSuccess ||
+                hipEventElapsedTime(&ms, d->ev0, d->ev1) != hipSuccess) {
+                set_error("autotune timing failed");
+                return 0.0f;
+            }
+            if (rep > 0 && (t == 0.0f || ms < t)) t = ms;
+        }
+        if (t > 0.0f && (best == 0.0f || t < best)) {
+            best = t;
+            best_plan = c;
+        }
+    }
+    d->tuned[std::make_tuple(p0.variant, N, N, (int)p0.comp, (int)p0.newton)] = best_plan;
+    return best;
+}
+
+int nbody_hip_kernel_timer(int enable)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    d->timing = enable != 0;
+    if (enable) d->timer_used = 0;
+    return OK;
+}
+
+int nbody_hip_kernel_timer_read(double* total_ms, int* launches)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    double sum = 0.0;
+    for (size_t k = 0; k < d->timer_used; ++k) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventSynchronize(d->timer_events[k].second));
+        HIP_TRY(hipEventElapsedTime(&ms, d->timer_events[k].first, d->timer_events[k].second));
+        sum += ms;
+    }
+    if (total_ms) *total_ms = sum;
+    if (launches) *launches = (int)d->timer_used;
+    return OK;
+}
+
+const char* nbody_hip_last_error(void) { return g_last_error.c_str(); }
+
+int nbody_hip_abi_version(void) { return NBODY_HIP_ABI_VERSION; }
+
+int nbody_hip_device_info(int* compute_units, int* clock_khz, char* arch, size_t arch_len)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    if (compute_units) *compute_units = d->cus;
+    if (clock_khz) *clock_khz = d->clock_khz;
+    if (arch && arch_len) snprintf(arch, arch_len, "%s", d->arch);
+    return OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Device-side building blocks shared by every force kernel (gfx950, wave64).
+//
+// The interaction is the reference's bodyBodyInteraction
+// (src/bodybodyInteraction.cuh:40-65): f_i += m_j (x_j - x_i) / (|x_j - x_i|^2 + eps2)^(3/2),
+// without m_i and without G. It is written here as 12 VALU ops + 1 v_rsq_f32:
+//   3 v_sub (d = x_j - x_i), 3 v_fma (d2 = eps2 + dx^2 + dy^2 + dz^2),
+//   v_rsq_f32, 3 v_mul (s = (m_j inv) (inv inv)), 3 v_fma (a += d s).
+// Differences from the reference's FP32 rounding sequence, all within rounding:
+// eps2 enters the FMA chain first, the sums are fused, and the product is
+// grouped (m inv)(inv inv). v_rsq_f32 measures <= 0.86 ulp on gfx950, so the
+// optional Newton step (NEWTON) is off by default.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nbody {
+
+// Knuth's TwoSum: returns s = fl(a + b) and sets err so that s + err == a + b
+// exactly (round-to-nearest, no overflow). Six adds; no reassociation is
+// allowed in this translation unit (no fast-math).
+__device__ __forceinline__ float two_sum(float a, float b, float& err)
+{
+    const float s = a + b;
+    const float bv = s - a;
+    const float av = s - bv;
+    err = (a - av) + (b - bv);
+    return s;
+}
+
+template <bool NEWTON>
+__device__ __forceinline__ float rsqrt_f32(float d2)
+{
+    float y = __builtin_amdgcn_rsqf(d2);
+    if constexpr (NEWTON) {
+        // one Newton-Raphson step in residual form: y += (y/2)(1 - d2 y^2)
+        const float e = __builtin_fmaf(-(d2 * y), y, 1.0f);
+        y = __builtin_fmaf(0.5f * y, e, y);
+    }
+    return y;
+}
+
+template <bool NEWTON>
+__device__ __forceinline__ void body_body(float xj, float yj, float zj, float mj,
+                                          float xi, float yi, float zi, float eps2,
+                                          float& ax, float& ay, float& az)
+{
+    const float dx = xj - xi;
+    const float dy = yj - yi;
+    const float dz = zj - zi;
+    float d2 = __builtin_fmaf(dx, dx, eps2);
+    d2 = __builtin_fmaf(dy, dy, d2);
+    d2 = __builtin_fmaf(dz, dz, d2);
+    const float inv = rsqrt_f32<NEWTON>(d2);
+    const float s = (mj * inv) * (inv * inv);
+    ax = __builtin_fmaf(dx, s, ax);
+    ay = __builtin_fmaf(dy, s, ay);
+    az = __builtin_fmaf(dz, s, az);
+}
+
+// DPP row_newbcast:K (gfx90a and later): every lane of each 16-lane row reads
+// lane K of its row. The combine pass folds it into the consuming VOP2
+// (v_sub_f32_dpp / v_mul_f32_dpp), so the broadcast costs no extra VALU op.
+template <int K>
+__device__ __forceinline__ float row_bcast(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x150 + K, 0xf, 0xf, false));
+}
+
+}  // namespace nbody

@@ -1,0 +1,67 @@
+// Host-visible description of the force kernels and their launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace nbody {
+
+// How the wave-uniform j-body reaches the 64 lanes of a wave.
+enum Variant : int {
+    VAR_AOS = 0,        // one i per lane over the whole j range (the ladder's first rung)
+    VAR_LDS = 1,        // 256-body j tiles staged in LDS, broadcast ds_read_b128
+    VAR_DPP = 2,        // 16 j per row in VGPRs, DPP row_newbcast into the VALU op
+    VAR_SGPR = 3,       // scalar loads: j lives in SGPRs, used as VALU operands
+    VAR_SYMMETRIC = 4,  // each pair once, reaction kept in LDS (no global atomics)
+    VAR_COUNT = 5
+};
+
+// One launch's decomposition: i bodies per lane, j range split into slices
+// whose partial sums are reduced in slice order (deterministic).
+struct ForcePlan {
+    int variant = VAR_SGPR;
+    int ipl = 1;        // i bodies per lane: 1, 2 or 4
+    int jslice = 0;     // j bodies per slice
+    int nslices = 1;    // ceil(n / jslice)
+    bool comp = false;  // TwoSum-compensated accumulation
+    bool newton = false;
+};
+
+struct ForceParams {
+    const float4* pm = nullptr;  // posMass (x, y, z, m) of the whole system: j source and i source
+    int n = 0;                   // bodies in the system
+    int i0 = 0;                  // first i body
+    int ni = 0;                  // number of i bodies; outputs are local: force[4 * (i - i0)]
+    float eps2 = 0.f;
+    int jslice = 0;
+    int nslices = 1;
+    float4* hi = nullptr;        // slice partials [nslices][ni]; nullptr -> write force directly
+    float4* lo = nullptr;        // compensation partials [nslices][ni] (comp only)
+    float* force = nullptr;      // 4 floats per i body; x, y, z written, w left alone
+};
+
+// Bytes of slice-partial scratch a plan needs for ni bodies (0 when direct).
+size_t plan_scratch_bytes(const ForcePlan& plan, int ni);
+
+// Queue the force kernel and, for sliced plans, the slice reduction on
+// `stream`. When before/after are given they are recorded on the stream
+// around the force kernel alone.
+hipError_t launch_force(const ForcePlan& plan, const ForceParams& p, hipStream_t stream,
+                        hipEvent_t before = nullptr, hipEvent_t after = nullptr);
+
+// integrateGravitation (src/nbody.cu:158-236) on float4 records.
+hipError_t launch_integrate(float4* posMass, float4* velInvMass, const float4* force, float dt,
+                            float damping, int n, hipStream_t stream);
+
+// FP64 direct sum (diagnostic): out holds 4 doubles per i body (w untouched).
+// partial must hold nslices * ni * 3 doubles.
+hipError_t launch_force_fp64(double* out, const float4* pm, int n, int i0, int ni, double eps2,
+                             int jslice, double* partial, hipStream_t stream);
+
+// FP64 energy/momentum diagnostic: acc[0] kinetic, acc[1] potential,
+// acc[2..4] momentum; partial holds 5 * blocks doubles (see fp64_energy_blocks).
+int fp64_energy_blocks(int n);
+hipError_t launch_energy_fp64(double* acc, const float4* pm, const float4* vim, int n, double eps2,
+                              double* partial, hipStream_t stream);
+
+}  // namespace nbody

@@ -1,0 +1,510 @@
+// Force, integrate and FP64 diagnostic kernels for gfx950 (MI355X), wave64.
+//
+// Decomposition shared by the FP32 force kernels (every variant):
+//   * block = 256 lanes = 4 waves; lane t of block b owns i bodies
+//     m = b*256*I + u*256 + t (u < I), so each wave's i loads and stores are
+//     contiguous float4 runs;
+//   * blockIdx.y selects a j slice [y*jslice, min(n, (y+1)*jslice));
+//   * the variants differ only in how the wave-uniform j body reaches the
+//     lanes (scalar loads into SGPRs, LDS broadcast, DPP row broadcast);
+//   * slice partials (float4, plus a float4 of TwoSum residues when
+//     compensated) are summed by k_reduce in slice order, so a result depends
+//     on the plan only, never on scheduling.
+// With one slice the kernel writes force directly. Force records are written
+// x, y, z only; w is left alone (the contract of src/nbody.cu:351-371).
+#include "nbody_device.h"
+#include "nbody_kernels.h"
+
+namespace nbody {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kChunk = 128;  // j bodies between TwoSum folds (multiple of 16)
+
+template <int I>
+struct Lanes {
+    float x[I], y[I], z[I];
+};
+
+template <int I>
+__device__ __forceinline__ void load_i(const ForceParams& p, Lanes<I>& me)
+{
+    const int mbase = blockIdx.x * (kBlock * I) + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < I; ++u) {
+        const int m = min(mbase + u * kBlock, p.ni - 1);
+        const float4 q = p.pm[p.i0 + m];
+        me.x[u] = q.x;
+        me.y[u] = q.y;
+        me.z[u] = q.z;
+    }
+}
+
+template <int I>
+__device__ __forceinline__ void zero(Lanes<I>& a)
+{
+#pragma unroll
+    for (int u = 0; u < I; ++u) a.x[u] = a.y[u] = a.z[u] = 0.f;
+}
+
+// hi += part with TwoSum, residue into lo.
+template <int I>
+__device__ __forceinline__ void fold(Lanes<I>& hi, Lanes<I>& lo, const Lanes<I>& part)
+{
+#pragma unroll
+    for (int u = 0; u < I; ++u) {
+        float e;
+        hi.x[u] = two_sum(hi.x[u], part.x[u], e);
+        lo.x[u] += e;
+        hi.y[u] = two_sum(hi.y[u], part.y[u], e);
+        lo.y[u] += e;
+        hi.z[u] = two_sum(hi.z[u], part.z[u], e);
+        lo.z[u] += e;
+    }
+}
+
+template <int I>
+__device__ __forceinline__ void store(const ForceParams& p, const Lanes<I>& hi, const Lanes<I>* lo)
+{
+    const int mbase = blockIdx.x * (kBlock * I) + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < I; ++u) {
+        const int m = mbase + u * kBlock;
+        if (m >= p.ni) continue;
+        if (p.hi == nullptr) {
+            float* f = p.force + 4 * (size_t)m;
+            if (lo) {
+                f[0] = hi.x[u] + lo->x[u];
+                f[1] = hi.y[u] + lo->y[u];
+                f[2] = hi.z[u] + lo->z[u];
+            } else {
+                f[0] = hi.x[u];
+                f[1] = hi.y[u];
+                f[2] = hi.z[u];
+            }
+        } else {
+            const size_t k = (size_t)blockIdx.y * p.ni + m;
+            p.hi[k] = make_float4(hi.x[u], hi.y[u], hi.z[u], 0.f);
+            if (lo) p.lo[k] = make_float4(lo->x[u], lo->y[u], lo->z[u], 0.f);
+        }
+    }
+}
+
+// j bodies read with wave-uniform addresses: the compiler emits s_load_dwordx16
+// (four bodies per load) and every VALU op takes x_j, y_j, z_j, m_j as SGPR
+// operands, so no VALU or LDS work is spent on the broadcast.
+template <int I, bool NEWTON>
+__device__ __forceinline__ void sweep_sgpr(const float4* __restrict__ src, int len, float eps2,
+                                           const Lanes<I>& me, Lanes<I>& acc)
+{
+#pragma unroll 8
+    for (int j = 0; j < len; ++j) {
+        const float4 b = src[j];
+#pragma unroll
+        for (int u = 0; u < I; ++u)
+            body_body<NEWTON>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
+                              acc.z[u]);
+    }
+}
+
+template <int I, bool COMP, bool NEWTON>
+__global__ __launch_bounds__(kBlock) void k_force_sgpr(ForceParams p)
+{
+    Lanes<I> me, hi;
+    load_i<I>(p, me);
+    zero<I>(hi);
+    const int j0 = blockIdx.y * p.jslice;
+    const int len = min(p.n - j0, p.jslice);
+    const float4* __restrict__ src = p.pm + j0;
+    if constexpr (!COMP) {
+        sweep_sgpr<I, NEWTON>(src, len, p.eps2, me, hi);
+        store<I>(p, hi, nullptr);
+    } else {
+        Lanes<I> lo;
+        zero<I>(lo);
+        for (int c = 0; c < len; c += kChunk) {
+            Lanes<I> part;
+            zero<I>(part);
+            sweep_sgpr<I, NEWTON>(src + c, min(kChunk, len - c), p.eps2, me, part);
+            fold<I>(hi, lo, part);
+        }
+        store<I>(p, hi, &lo);
+    }
+}
+
+// The reference's first rung (src/nbody_GPU_AOS.cu:42-75): one i body per
+// lane, the whole j range, no slices.
+template <bool NEWTON>
+__global__ __launch_bounds__(kBlock) void k_force_aos(ForceParams p)
+{
+    Lanes<1> me, acc;
+    load_i<1>(p, me);
+    zero<1>(acc);
+    for (int j = 0; j < p.n; ++j) {
+        const float4 b = p.pm[j];
+        body_body<NEWTON>(b.x, b.y, b.z, b.w, me.x[0], me.y[0], me.z[0], p.eps2, acc.x[0], acc.y[0],
+                          acc.z[0]);
+    }
+    store<1>(p, acc, nullptr);
+}
+
+// j tiles of 256 bodies staged in LDS by the whole block (one coalesced float4
+// per lane), double-buffered so one barrier per tile suffices; each wave then
+// reads body k with a uniform address (a broadcast ds_read_b128).
+template <int I, bool COMP, bool NEWTON>
+__global__ __launch_bounds__(kBlock) void k_force_lds(ForceParams p)
+{
+    __shared__ float4 tile[2][kBlock];
+    Lanes<I> me, hi, lo;
+    load_i<I>(p, me);
+    zero<I>(hi);
+    zero<I>(lo);
+    const int j0 = blockIdx.y * p.jslice;
+    const int len = min(p.n - j0, p.jslice);
+    const float4* __restrict__ src = p.pm + j0;
+    int buf = 0;
+    for (int t = 0; t < len; t += kBlock, buf ^= 1) {
+        const int k = t + threadIdx.x;
+        tile[buf][threadIdx.x] = k < len ? src[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        const int cnt = min(kBlock, len - t);
+        Lanes<I> part;
+        zero<I>(part);
+        Lanes<I>& acc = COMP ? part : hi;
+#pragma unroll 8
+        for (int q = 0; q < cnt; ++q) {
+            const float4 b = tile[buf][q];
+#pragma unroll
+            for (int u = 0; u < I; ++u)
+                body_body<NEWTON>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], p.eps2, acc.x[u],
+                                  acc.y[u], acc.z[u]);
+        }
+        if constexpr (COMP) fold<I>(hi, lo, part);
+    }
+    store<I>(p, hi, COMP ? &lo : nullptr);
+}
+
+template <int I, bool NEWTON, int K>
+__device__ __forceinline__ void dpp_steps(const float4& b, float eps2, const Lanes<I>& me, Lanes<I>& acc)
+{
+    if constexpr (K < 16) {
+#pragma unroll
+        for (int u = 0; u < I; ++u)
+            body_body<NEWTON>(row_bcast<K>(b.x), row_bcast<K>(b.y), row_bcast<K>(b.z), row_bcast<K>(b.w),
+                              me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u], acc.z[u]);
+        dpp_steps<I, NEWTON, K + 1>(b, eps2, me, acc);
+    }
+}
+
+// Each 16-lane row loads the same 16 j bodies (one float4 per lane); step K
+// reads lane K of the row through DPP row_newbcast folded into the VALU op.
+// The wave64 counterpart of the reference's warp-shuffle broadcast
+// (src/nbody_GPU_Shuffle.cu:44-89). Bodies past the last full group of 16 go
+// through the scalar-load sweep.
+template <int I, bool NEWTON>
+__device__ __forceinline__ void sweep_dpp(const float4* __restrict__ src, int len, float eps2,
+                                          const Lanes<I>& me, Lanes<I>& acc)
+{
+    const int lane16 = threadIdx.x & 15;
+    int g = 0;
+    for (; g + 16 <= len; g += 16) {
+        const float4 b = src[g + lane16];
+        dpp_steps<I, NEWTON, 0>(b, eps2, me, acc);
+    }
+    if (g < len) sweep_sgpr<I, NEWTON>(src + g, len - g, eps2, me, acc);
+}
+
+template <int I, bool COMP, bool NEWTON>
+__global__ __launch_bounds__(kBlock) void k_force_dpp(ForceParams p)
+{
+    Lanes<I> me, hi;
+    load_i<I>(p, me);
+    zero<I>(hi);
+    const int j0 = blockIdx.y * p.jslice;
+    const int len = min(p.n - j0, p.jslice);
+    const float4* __restrict__ src = p.pm + j0;
+    if constexpr (!COMP) {
+        sweep_dpp<I, NEWTON>(src, len, p.eps2, me, hi);
+        store<I>(p, hi, nullptr);
+    } else {
+        Lanes<I> lo;
+        zero<I>(lo);
+        for (int c = 0; c < len; c += kChunk) {
+            Lanes<I> part;
+            zero<I>(part);
+            sweep_dpp<I, NEWTON>(src + c, min(kChunk, len - c), p.eps2, me, part);
+            fold<I>(hi, lo, part);
+        }
+        store<I>(p, hi, &lo);
+    }
+}
+
+// Slice partials summed in slice order; compensated plans merge the TwoSum
+// residues and round once at the end.
+template <bool COMP>
+__global__ __launch_bounds__(kBlock) void k_reduce(const float4* __restrict__ hi, const float4* __restrict__ lo,
+                                                   int nslices, int ni, float* __restrict__ force)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= ni) return;
+    float4 a = hi[m];
+    float4 c = COMP ? lo[m] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int s = 1; s < nslices; ++s) {
+        const size_t k = (size_t)s * ni + m;
+        const float4 h = hi[k];
+        if constexpr (COMP) {
+            const float4 l = lo[k];
+            float e;
+            a.x = two_sum(a.x, h.x, e);
+            c.x += e + l.x;
+            a.y = two_sum(a.y, h.y, e);
+            c.y += e + l.y;
+            a.z = two_sum(a.z, h.z, e);
+            c.z += e + l.z;
+        } else {
+            a.x += h.x;
+            a.y += h.y;
+            a.z += h.z;
+        }
+    }
+    float* f = force + 4 * (size_t)m;
+    f[0] = a.x + c.x;
+    f[1] = a.y + c.y;
+    f[2] = a.z + c.z;
+}
+
+// src/nbody.cu:158-182 integrateOne, in the operation order the reference's
+// -ffast-math CPU build executes it: v = fma(invMass, f*dt, v); v *= damping;
+// x = fma(v, dt, x). Mass (posMass.w) and invMass (velInvMass.w) are kept.
+__global__ __launch_bounds__(kBlock) void k_integrate(float4* __restrict__ pm, float4* __restrict__ vim,
+                                                      const float4* __restrict__ f, float dt, float damping, int n)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pm[i];
+    float4 v = vim[i];
+    const float4 g = f[i];
+    v.x = __builtin_fmaf(v.w, g.x * dt, v.x);
+    v.y = __builtin_fmaf(v.w, g.y * dt, v.y);
+    v.z = __builtin_fmaf(v.w, g.z * dt, v.z);
+    v.x *= damping;
+    v.y *= damping;
+    v.z *= damping;
+    p.x = __builtin_fmaf(v.x, dt, p.x);
+    p.y = __builtin_fmaf(v.y, dt, p.y);
+    p.z = __builtin_fmaf(v.z, dt, p.z);
+    pm[i] = p;
+    vim[i] = v;
+}
+
+// FP64 direct sum, one i body per lane over one j slice (the j body again
+// comes from scalar loads); IEEE division and square root.
+__global__ __launch_bounds__(kBlock) void k_force_fp64(const float4* __restrict__ pm, int n, int i0, int ni,
+                                                       double eps2, int jslice, double* __restrict__ partial)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    const float4 q = pm[i0 + min(m, ni - 1)];
+    const double xi = q.x, yi = q.y, zi = q.z;
+    const int j0 = blockIdx.y * jslice;
+    const int len = min(n - j0, jslice);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int j = 0; j < len; ++j) {
+        const float4 b = pm[j0 + j];
+        const double dx = (double)b.x - xi, dy = (double)b.y - yi, dz = (double)b.z - zi;
+        const double d2 = dx * dx + dy * dy + dz * dz + eps2;
+        const double inv = 1.0 / sqrt(d2);
+        const double s = (double)b.w * (inv * inv * inv);
+        ax += dx * s;
+        ay += dy * s;
+        az += dz * s;
+    }
+    if (m < ni) {
+        double* o = partial + 3 * ((size_t)blockIdx.y * ni + m);
+        o[0] = ax;
+        o[1] = ay;
+        o[2] = az;
+    }
+}
+
+__device__ __forceinline__ void neumaier(double& s, double& c, double x)
+{
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_fp64(const double* __restrict__ partial, int nslices, int ni,
+                                                        double* __restrict__ out)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= ni) return;
+    for (int c = 0; c < 3; ++c) {
+        double s = 0.0, e = 0.0;
+        for (int k = 0; k < nslices; ++k) neumaier(s, e, partial[3 * ((size_t)k * ni + m) + c]);
+        out[4 * (size_t)m + c] = s + e;
+    }
+}
+
+// Energy diagnostic, pass 1: block b reduces bodies [256b, 256b+256):
+// kinetic 1/2 m v^2, momentum m v, and 1/2 sum_{j != i} -m_i m_j / sqrt(d^2 + eps2)
+// (each pair seen from both ends, hence the half). Tree sums in LDS, fixed order.
+__global__ __launch_bounds__(kBlock) void k_energy_fp64(const float4* __restrict__ pm, const float4* __restrict__ vim,
+                                                        int n, double eps2, double* __restrict__ partial)
+{
+    __shared__ double red[5][kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const float4 q = pm[i];
+        const float4 w = vim[i];
+        const double m = q.w;
+        v[0] = 0.5 * m * ((double)w.x * w.x + (double)w.y * w.y + (double)w.z * w.z);
+        v[2] = m * w.x;
+        v[3] = m * w.y;
+        v[4] = m * w.z;
+        double s = 0.0, e = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const float4 b = pm[j];
+            const double dx = (double)b.x - q.x, dy = (double)b.y - q.y, dz = (double)b.z - q.z;
+            const double t = j == i ? 0.0 : (double)b.w / sqrt(dx * dx + dy * dy + dz * dz + eps2);
+            neumaier(s, e, t);
+        }
+        v[1] = -0.5 * m * (s + e);
+    }
+    for (int c = 0; c < 5; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int c = 0; c < 5; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 5; ++c) partial[5 * (size_t)blockIdx.x + c] = red[c][0];
+}
+
+__global__ void k_energy_fp64_final(const double* __restrict__ partial, int blocks, double* __restrict__ acc)
+{
+    const int c = threadIdx.x;
+    if (c >= 5) return;
+    double s = 0.0, e = 0.0;
+    for (int b = 0; b < blocks; ++b) neumaier(s, e, partial[5 * (size_t)b + c]);
+    acc[c] = s + e;
+}
+
+#define NBODY_KERNEL_FAMILY(NAME, KERNEL)                                                        \
+    struct NAME {                                                                                \
+        template <int I, bool C, bool N>                                                         \
+        static void go(dim3 g, hipStream_t s, const ForceParams& p)                              \
+        {                                                                                        \
+            KERNEL<I, C, N><<<g, kBlock, 0, s>>>(p);                                             \
+        }                                                                                        \
+    };
+NBODY_KERNEL_FAMILY(FamSgpr, k_force_sgpr)
+NBODY_KERNEL_FAMILY(FamLds, k_force_lds)
+NBODY_KERNEL_FAMILY(FamDpp, k_force_dpp)
+#undef NBODY_KERNEL_FAMILY
+
+template <class Fam, int I>
+void go_i(const ForcePlan& plan, dim3 g, hipStream_t s, const ForceParams& p)
+{
+    if (plan.comp) {
+        if (plan.newton) Fam::template go<I, true, true>(g, s, p);
+        else Fam::template go<I, true, false>(g, s, p);
+    } else {
+        if (plan.newton) Fam::template go<I, false, true>(g, s, p);
+        else Fam::template go<I, false, false>(g, s, p);
+    }
+}
+
+template <class Fam>
+hipError_t go_family(const ForcePlan& plan, dim3 g, hipStream_t s, const ForceParams& p)
+{
+    switch (plan.ipl) {
+    case 1: go_i<Fam, 1>(plan, g, s, p); break;
+    case 2: go_i<Fam, 2>(plan, g, s, p); break;
+    case 4: go_i<Fam, 4>(plan, g, s, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t plan_scratch_bytes(const ForcePlan& plan, int ni)
+{
+    if (plan.nslices <= 1) return 0;
+    return (size_t)plan.nslices * ni * sizeof(float4) * (plan.comp ? 2 : 1);
+}
+
+hipError_t launch_force(const ForcePlan& plan, const ForceParams& p, hipStream_t stream, hipEvent_t before,
+                        hipEvent_t after)
+{
+    if (p.ni <= 0 || p.n <= 0) return hipSuccess;
+    if (before) {
+        hipError_t e = hipEventRecord(before, stream);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e;
+    if (plan.variant == VAR_AOS) {
+        const dim3 g((p.ni + kBlock - 1) / kBlock, 1);
+        ForceParams q = p;
+        q.hi = q.lo = nullptr;
+        if (plan.newton) k_force_aos<true><<<g, kBlock, 0, stream>>>(q);
+        else k_force_aos<false><<<g, kBlock, 0, stream>>>(q);
+        e = hipGetLastError();
+        if (e == hipSuccess && after) e = hipEventRecord(after, stream);
+        return e;
+    }
+    const int per_block = kBlock * plan.ipl;
+    const dim3 g((p.ni + per_block - 1) / per_block, p.nslices);
+    switch (plan.variant) {
+    case VAR_SGPR: e = go_family<FamSgpr>(plan, g, stream, p); break;
+    case VAR_LDS: e = go_family<FamLds>(plan, g, stream, p); break;
+    case VAR_DPP: e = go_family<FamDpp>(plan, g, stream, p); break;
+    default: return hipErrorNotSupported;
+    }
+    if (e == hipSuccess && after) e = hipEventRecord(after, stream);
+    if (e != hipSuccess || p.hi == nullptr) return e;
+    const dim3 gr((p.ni + kBlock - 1) / kBlock);
+    if (plan.comp) k_reduce<true><<<gr, kBlock, 0, stream>>>(p.hi, p.lo, p.nslices, p.ni, p.force);
+    else k_reduce<false><<<gr, kBlock, 0, stream>>>(p.hi, p.lo, p.nslices, p.ni, p.force);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate(float4* posMass, float4* velInvMass, const float4* force, float dt, float damping, int n,
+                            hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    k_integrate<<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(posMass, velInvMass, force, dt, damping, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_fp64(double* out, const float4* pm, int n, int i0, int ni, double eps2, int jslice,
+                             double* partial, hipStream_t stream)
+{
+    if (ni <= 0 || n <= 0) return hipSuccess;
+    const int nslices = (n + jslice - 1) / jslice;
+    k_force_fp64<<<dim3((ni + kBlock - 1) / kBlock, nslices), kBlock, 0, stream>>>(pm, n, i0, ni, eps2, jslice,
+                                                                                  partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_reduce_fp64<<<(ni + kBlock - 1) / kBlock, kBlock, 0, stream>>>(partial, nslices, ni, out);
+    return hipGetLastError();
+}
+
+int fp64_energy_blocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+hipError_t launch_energy_fp64(double* acc, const float4* pm, const float4* vim, int n, double eps2, double* partial,
+                              hipStream_t stream)
+{
+    if (n <= 0) return hipErrorInvalidValue;
+    const int blocks = fp64_energy_blocks(n);
+    k_energy_fp64<<<blocks, kBlock, 0, stream>>>(pm, vim, n, eps2, partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_energy_fp64_final<<<1, 64, 0, stream>>>(partial, blocks, acc);
+    return hipGetLastError();
+}
+
+}  // namespace nbody

@@ -1,0 +1,209 @@
+"""Parity of the HIP force path (through the C ABI) with the oracle.
+
+Tolerances (FP32 kernels against the FP64 oracle, per-body normwise
+|F - F64| / |F64|): max <= 1e-5 and RMS <= 1e-6 for every variant and size;
+against the reference's FP32 CPU_AOS crosscheck: max <= 2e-5 (the sum of the
+two FP32 errors). Integer-like properties (w untouched, determinism, i-range
+equality, power-of-two mass scaling) are bit-exact.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import EPS2
+from gpu_util import dev, force_buffer, host, ptr, run_entry, run_force
+from oracle_lib import force_errors
+
+from nbody_amd import _native, ics
+
+pytestmark = pytest.mark.gpu
+
+TOL_MAX, TOL_RMS, TOL_AOS = 1e-5, 1e-6, 2e-5
+ENTRIES = ["ComputeGravitation_HIP_AOS", "ComputeGravitation_HIP_LDS", "ComputeGravitation_HIP_DPP",
+           "ComputeGravitation_HIP_SGPR", "ComputeGravitation_HIP"]
+SIZES = [1, 2, 3, 17, 255, 256, 257, 1000, 1024, 4097, 16384]
+
+
+def bodies(n, seed=7):
+    pm, vm = ics.random_unit_bodies(n, seed=seed)
+    return pm, vm
+
+
+def check_against_oracle(oracle, pm, f, what):
+    n = pm.shape[0]
+    assert np.all(f[:, 3] == -7.0), f"{what}: force.w was written"
+    assert np.all(np.isfinite(f[:, :3])), f"{what}: unwritten or non-finite force"
+    if n == 1:
+        assert np.all(f[:, :3] == 0.0)
+        return
+    f64 = oracle.fp64_forces(pm, EPS2)
+    mx, rms = force_errors(f, f64)
+    assert mx <= TOL_MAX and rms <= TOL_RMS, (what, n, mx, rms)
+    aos, _ = oracle.cpu_aos(pm, EPS2)
+    d = np.linalg.norm(f[:, :3].astype(np.float64) - aos[:, :3], axis=1) / np.linalg.norm(f64, axis=1)
+    assert d.max() <= TOL_AOS, (what, n, d.max())
+
+
+@pytest.mark.parametrize("n", SIZES)
+@pytest.mark.parametrize("entry", ENTRIES)
+def test_entry_matches_oracle(oracle, entry, n):
+    pm, _ = bodies(n)
+    f, ms = run_entry(entry, dev(pm), EPS2)
+    assert ms > 0
+    check_against_oracle(oracle, pm, f, entry)
+
+
+@pytest.mark.parametrize("variant", [_native.SGPR, _native.LDS, _native.DPP])
+@pytest.mark.parametrize("ipl", [1, 2, 4])
+@pytest.mark.parametrize("comp", [0, 1])
+@pytest.mark.parametrize("newton", [0, 1])
+def test_plan_options(oracle, variant, ipl, comp, newton):
+    pm, _ = bodies(5003, seed=2)
+    d = dev(pm)
+    f64 = oracle.fp64_forces(pm, EPS2)
+    for jslice in (256, 1000, 5003):
+        f = run_force(d, EPS2, variant=variant, i_per_lane=ipl, jslice=jslice, compensated=comp, newton=newton)
+        mx, rms = force_errors(f, f64)
+        assert mx <= TOL_MAX and rms <= TOL_RMS, (variant, ipl, jslice, comp, newton, mx, rms)
+        assert np.all(f[:, 3] == -7.0)
+
+
+@pytest.mark.parametrize("variant", [_native.SGPR, _native.LDS, _native.DPP])
+def test_deterministic(variant):
+    pm, _ = bodies(20000, seed=4)
+    d = dev(pm)
+    a = run_force(d, EPS2, variant=variant, compensated=1)
+    b = run_force(d, EPS2, variant=variant, compensated=1)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("variant", [_native.SGPR, _native.LDS, _native.DPP])
+def test_i_range_equals_full(variant):
+    """The base/n entry (the multi-GPU i-slice of src/nbody_multiGPU_shared.cuh)
+    gives each body exactly the force of the full evaluation under one plan."""
+    pm, _ = bodies(9000, seed=5)
+    d = dev(pm)
+    opt = dict(variant=variant, i_per_lane=2, jslice=1024, compensated=1)
+    full = run_force(d, EPS2, **opt)
+    for base, count in [(0, 1), (4500, 1125), (8999, 1), (1, 8999), (2048, 2048)]:
+        part = run_force(d, EPS2, base=base, count=count, **opt)
+        assert np.array_equal(part.view(np.uint32), full[base:base + count].view(np.uint32)), (base, count)
+
+
+def test_mass_scaling_is_exact():
+    pm, _ = bodies(4096, seed=6)
+    a = run_force(dev(pm), EPS2, variant=_native.SGPR)
+    pm2 = pm.copy()
+    pm2[:, 3] *= 2.0
+    b = run_force(dev(pm2), EPS2, variant=_native.SGPR)
+    assert np.array_equal(b[:, :3], 2.0 * a[:, :3])
+
+
+def test_host_twin_matches_device_entry():
+    pm, _ = bodies(3000, seed=8)
+    f_dev, _ = run_entry("ComputeGravitation_HIP", dev(pm), EPS2)
+    fh = np.full((3000, 4), np.nan, np.float32)
+    fh[:, 3] = -7.0
+    ms = _native.lib().ComputeGravitation_HIP_Host(fh.ctypes.data, np.ascontiguousarray(pm).ctypes.data, EPS2, 3000)
+    assert ms > 0
+    assert np.array_equal(fh.view(np.uint32), f_dev.view(np.uint32))
+
+
+def test_reference_fixture_crosscheck(oracle):
+    """seed-7 ICs of the reference driver against the forces its own CPU_SOA
+    build produced (tests/golden), using the reference's crosscheck metric
+    (src/nbody.cu:406-418) and the normwise one."""
+    import os
+    from conftest import GOLDEN
+    g = np.load(os.path.join(GOLDEN, "ref_seed7_n4096.npz"))
+    pm = g["pos_mass"]
+    f, _ = run_entry("ComputeGravitation_HIP", dev(pm), EPS2)
+    golden = np.zeros((4096, 4), np.float32)
+    golden[:, :3] = g["force_cpu_soa"]
+    rel = oracle.max_rel_error(f, golden)
+    f64 = oracle.fp64_forces(pm, EPS2)
+    d = np.linalg.norm(f[:, :3].astype(np.float64) - golden[:, :3], axis=1) / np.linalg.norm(f64, axis=1)
+    print(f"reference-style max relError {rel:.3e}, normwise max {d.max():.3e}")
+    assert d.max() <= TOL_AOS
+
+
+def test_integrate_bitexact_vs_oracle(oracle):
+    n = 5000
+    pm, vm = bodies(n, seed=9)
+    rng = np.random.default_rng(0)
+    f = rng.normal(size=(n, 4)).astype(np.float32)
+    dpm, dvm, df = dev(pm), dev(vm), dev(f)
+    ms = _native.lib().integrateGravitation_HIP(ptr(dpm), ptr(dvm), ptr(df), 0.016, 0.95, n)
+    assert ms > 0
+    p1, v1 = oracle.integrate_aos(pm, vm, f, 0.016, 0.95)
+    assert np.array_equal(host(dpm).view(np.uint32), p1.view(np.uint32))
+    assert np.array_equal(host(dvm).view(np.uint32), v1.view(np.uint32))
+
+
+def test_fp64_force_matches_oracle(oracle):
+    pm, _ = bodies(3001, seed=10)
+    out = dev(np.full((3001, 4), -9.0, np.float64))
+    ms = _native.lib().nbody_hip_force_fp64(ptr(out), ptr(dev(pm)), EPS2, 3001)
+    assert ms > 0
+    o = host(out)
+    f64 = oracle.fp64_forces(pm, EPS2)
+    assert np.all(o[:, 3] == -9.0)
+    err = np.linalg.norm(o[:, :3] - f64, axis=1) / np.linalg.norm(f64, axis=1)
+    assert err.max() < 1e-12
+
+
+def test_energy_fp64_matches_oracle(oracle):
+    pm, vm = bodies(2000, seed=11)
+    out = np.zeros(5, np.float64)
+    rc = _native.lib().nbody_hip_energy_fp64(out.ctypes.data, ptr(dev(pm)), ptr(dev(vm)), EPS2, 2000)
+    _native.check(rc, "energy")
+    ke, pe, mom = oracle.fp64_energy(pm, vm, EPS2)
+    assert out[0] == pytest.approx(ke, rel=1e-12)
+    assert out[1] == pytest.approx(pe, rel=1e-10)
+    assert out[2:] == pytest.approx(mom, rel=1e-10, abs=1e-10 * np.abs(mom).max())
+
+
+def test_bad_arguments_fail_loudly():
+    L = _native.lib()
+    pm, _ = bodies(16)
+    d = dev(pm)
+    f = force_buffer(16)
+    assert L.ComputeGravitation_HIP_SGPR(ptr(f), ptr(d), EPS2, 0) == 0.0
+    assert "N = 0" in _native.last_error()
+    assert L.ComputeGravitation_HIP_SGPR(None, ptr(d), EPS2, 16) == 0.0
+    o = _native.Options.make(variant=_native.SGPR, i_per_lane=3)
+    assert L.nbody_hip_force(ptr(f), ptr(d), EPS2, 16, 0, 16, ctypes.byref(o), None) != 0
+    assert "i_per_lane" in _native.last_error()
+    assert L.nbody_hip_force(ptr(f), ptr(d), EPS2, 16, 10, 7, None, None) != 0
+    assert np.all(np.isnan(host(f)[:, :3]))  # nothing ran
+
+
+def test_autotune_then_default(oracle):
+    pm, _ = bodies(12000, seed=12)
+    d = dev(pm)
+    f = force_buffer(12000)
+    best = _native.lib().nbody_hip_autotune(ptr(f), ptr(d), EPS2, 12000, _native.AUTO)
+    _native.check_ms(best, "autotune")
+    g, _ = run_entry("ComputeGravitation_HIP", d, EPS2)
+    mx, rms = force_errors(g, oracle.fp64_forces(pm, EPS2))
+    assert mx <= TOL_MAX and rms <= TOL_RMS
+
+
+@pytest.mark.slow
+def test_full_size_98304_sampled(oracle):
+    """The single-system workload (N = 98304 uniform ball, eps^2 = 0.01): the
+    default entry against FP64 on a sample that includes the 256 bodies
+    nearest the centre (smallest |F|, largest relative error), and the
+    size-independent equal-mass property sum_i F_i = 0."""
+    n = 98304
+    pm, _ = ics.uniform_ball(n, seed=1)
+    f, ms = run_entry("ComputeGravitation_HIP", dev(pm), 0.01)
+    r = np.linalg.norm(pm[:, :3], axis=1)
+    idx = np.unique(np.concatenate([np.argsort(r)[:256], np.random.default_rng(0).choice(n, 1792, replace=False)]))
+    f64 = oracle.fp64_forces(pm, 0.01, idx)
+    mx, rms = force_errors(f[idx], f64)
+    print(f"N=98304 HIP: {ms:.3f} ms, sampled max rel {mx:.3e} rms {rms:.3e}")
+    assert mx <= TOL_MAX and rms <= TOL_RMS
+    tot = f[:, :3].astype(np.float64).sum(0)
+    assert np.abs(tot).max() <= 1e-6 * np.abs(f[:, :3]).sum()

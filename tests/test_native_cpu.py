@@ -1,0 +1,126 @@
+"""Product host code on the CPU: the C-ABI library loads and exports every
+symbol include/*.h declares (no GPU call is made), the initial-condition
+generators, and the algorithm table."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+import nbody_amd
+from nbody_amd import _native, algorithms, ics
+
+
+def header_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                             text, re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_entry_points():
+    names = header_functions()
+    for must in ("ComputeGravitation_HIP_SGPR", "integrateGravitation_HIP", "nbody_hip_force",
+                 "nbody_random_unit_bodies"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _native.lib()  # raises when the library is missing: no fallback
+    missing = [n for n in header_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    assert set(header_functions()) <= set(_native.SIGNATURES)
+
+
+def test_abi_version():
+    assert _native.lib().nbody_hip_abi_version() == _native.ABI_VERSION
+
+
+def test_missing_library_is_an_error(monkeypatch, tmp_path):
+    monkeypatch.setattr(_native, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(_native, "_lib", None)
+    with pytest.raises(_native.NativeLibraryMissing):
+        _native.lib()
+
+
+@pytest.mark.parametrize("n", [1, 1024, 4096, 98304])
+def test_random_unit_bodies_bitexact_vs_oracle(oracle, n):
+    pm, vm = ics.random_unit_bodies(n, seed=7)
+    po, vo = oracle.random_unit_bodies(n, 7)
+    assert np.array_equal(pm.view(np.uint32), po.view(np.uint32))
+    assert np.array_equal(vm.view(np.uint32), vo.view(np.uint32))
+
+
+def test_random_unit_bodies_other_seed_differs(oracle):
+    a, _ = ics.random_unit_bodies(1024, seed=8)
+    b, _ = oracle.random_unit_bodies(1024, 8)
+    c, _ = ics.random_unit_bodies(1024, seed=7)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+def test_uniform_ball_statistics():
+    n = 16384
+    pm, vm = ics.uniform_ball(n, seed=1)
+    assert ics.ball_radius(16384) == pytest.approx(1.54)
+    assert ics.ball_radius(98304) == pytest.approx(1.54 * 6 ** (1 / 3))
+    r = np.linalg.norm(pm[:, :3].astype(np.float64), axis=1) / 1.54
+    assert r.max() <= 1 + 1e-6
+    # uniform density: P(r < x) = x^3, so E[r^3] = 1/2 and the median radius is 2^(-1/3)
+    assert abs(np.mean(r ** 3) - 0.5) < 0.01
+    assert abs(np.median(r) - 2 ** (-1 / 3)) < 0.01
+    assert np.all(pm[:, 3] == 1.0) and np.all(vm[:, 3] == 1.0)
+    v = np.linalg.norm(vm[:, :3].astype(np.float64), axis=1)
+    assert v.max() <= 1 + 1e-6 and abs(np.mean(v ** 3) - 0.5) < 0.01
+    assert abs(pm[:, :3].mean(0)).max() < 0.03
+
+
+def test_plummer_statistics(oracle):
+    n = 8192
+    pm, vm = ics.plummer(n, seed=3)
+    assert pm[:, 3].sum() == pytest.approx(1.0, rel=1e-5)
+    assert np.all(vm[:, 3] == 1.0)
+    r = np.linalg.norm(pm[:, :3].astype(np.float64), axis=1)
+    # M(<a)/M = a^3/(a^2+a^2)^(3/2) = 2^(-3/2); M(<3a)/M = 27/10^(3/2)
+    for x, frac in [(1.0, 2 ** -1.5), (3.0, 27 / 10 ** 1.5)]:
+        assert abs(np.mean(r < x) - frac) < 0.02
+    assert r.max() <= 10.0 * (1 + 1e-6)
+    # virial equilibrium 2K + W = 0 (G = 1, softening small next to a = 1)
+    ke, pe, _ = oracle.fp64_energy(pm, np.column_stack([vm[:, :3], np.full(n, 1.0, np.float32)]), 1e-6)
+    assert abs(2 * ke / -pe - 1.0) < 0.08
+    pm2, _ = ics.plummer(n, seed=3, unit_inv_mass=False)
+    assert np.array_equal(pm2, pm)
+
+
+def test_plummer_inv_mass_option():
+    _, vm = ics.plummer(1000, seed=3, unit_inv_mass=False)
+    assert np.allclose(vm[:, 3], 1000.0)
+
+
+def test_log_uniform_sizes():
+    s = ics.log_uniform_sizes(512, 256, 16384, seed=4)
+    assert s.min() >= 256 and s.max() <= 16384 and len(s) == 512
+    l = np.log(s)
+    assert abs(np.median(l) - 0.5 * (np.log(256) + np.log(16384))) < 0.3
+
+
+def test_algorithm_table():
+    names = [a.name for a in algorithms.ALGORITHMS]
+    assert len(set(names)) == len(names)
+    for a in algorithms.ALGORITHMS:
+        assert hasattr(_native.lib(), a.symbol)
+    assert algorithms.find("hip_sgpr").name == "HIP_SGPR"
+    assert algorithms.find("HIP_Sym").name == "HIP_Symmetric"
+    with pytest.raises(KeyError):
+        algorithms.find("GPU_Shared")
+    assert nbody_amd.FLOPS_PER_INTERACTION == 20
