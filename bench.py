@@ -31,7 +31,6 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector / dense matrix peak (published)
-PEAK_FP32_NONPACKED = 78.65    # the same without packed-FP32 instructions (half rate)
 REF_BEST_GPU = 501.5           # 10^9 int/s, GPU_Shared clang build, RTX 2080 Ti, N=98304 (README.md:230-233)
 METRIC = "10^9 interactions/s + % FP32 peak at N=96K; ensemble throughput + p99 latency"
 
@@ -40,10 +39,15 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--n", type=int, default=98304)
-    ap.add_argument("--variant", default="auto", choices=["auto", "sgpr", "lds", "dpp", "aos"])
+    ap.add_argument("--variant", default="auto", choices=["auto", "sgpr", "lds", "dpp", "aos", "symmetric"])
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--ipl", type=int, default=0, help="pin i bodies per lane (skips the autotuner)")
+    ap.add_argument("--jslice", type=int, default=0, help="pin the j slice length (skips the autotuner)")
+    ap.add_argument("--compensated", type=int, default=-1, choices=[-1, 0, 1])
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (counter calibration for --pmc runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     return ap.parse_args()
@@ -90,6 +94,21 @@ def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s):
     }
 
 
+KERNEL_NAMES = {0: "k_force_aos", 1: "k_force_lds", 2: "k_force_dpp", 3: "k_force_sgpr", 4: "k_force_sym"}
+
+
+def pair_evaluations(plan, n):
+    """Interaction evaluations one launch executes. The symmetric kernel
+    evaluates each off-diagonal block pair once for both bodies and the
+    diagonal blocks in full; the others evaluate all N^2. The roofline is
+    credited with N^2 (20 flops each) either way."""
+    if plan.variant != 4:
+        return n * n
+    b = plan.jslice
+    nb = -(-n // b)
+    return nb * (nb - 1) // 2 * b * b + nb * b * b
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -125,6 +144,16 @@ def main():
     sys.path.insert(0, HERE)
     from nbody_amd import _native, ics
 
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
+                          "launches": 5}))
+        return
+
     L = _native.lib()
     n, K, W = args.n, args.steps, args.warmup
     eps2, dt, damping = 0.01, 0.016, 0.95
@@ -135,11 +164,13 @@ def main():
     force = torch.zeros((n, 4), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
     variant = {"auto": _native.AUTO, "sgpr": _native.SGPR, "lds": _native.LDS, "dpp": _native.DPP,
-               "aos": _native.AOS}[args.variant]
-    opt = _native.Options.make(variant=variant)
+               "aos": _native.AOS, "symmetric": _native.SYMMETRIC}[args.variant]
+    opt = _native.Options.make(variant=variant, i_per_lane=args.ipl, jslice=args.jslice,
+                               compensated=args.compensated)
     P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    pinned = args.ipl > 0 or args.jslice > 0
 
-    if not args.no_autotune and variant != _native.AOS:
+    if not args.no_autotune and not pinned and variant != _native.AOS:
         _native.check_ms(L.nbody_hip_autotune(P(force), P(pm), eps2, n, variant), "autotune")
     plan = _native.Options()
     _native.check(L.nbody_hip_plan(n, n, ctypes.byref(opt), ctypes.byref(plan)), "plan")
@@ -213,10 +244,9 @@ def main():
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": traffic_from_profiles(workload),
             "kernel_ms": round(kernel_ms, 4),
-            "kernel": f"k_force_{_native.VARIANT_NAMES[plan.variant].lower()}",
+            "kernel": KERNEL_NAMES[plan.variant],
             "flops_per_launch": 20 * int(inter),
-            "peak_without_packed_fp32": PEAK_FP32_NONPACKED,
-            "frac_without_packed_fp32": round(achieved / PEAK_FP32_NONPACKED, 4),
+            "pair_evaluations_per_launch": pair_evaluations(plan, n),
         },
         "accuracy": {"max_rel": max_rel, "rms_rel": rms_rel,
                      "vs": "FP64 direct sum (GPU diagnostic), per-body |F-F64|/|F64|"},

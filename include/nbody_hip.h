@@ -47,14 +47,17 @@ enum {
     NBODY_HIP_DPP = 2,       /* wave64 DPP row broadcast (src/nbody_GPU_Shuffle.cu:44-107) */
     NBODY_HIP_SGPR = 3,      /* scalar-memory broadcast (src/nbody_GPU_AOS_const.cu:45-121) */
     NBODY_HIP_SYMMETRIC = 4, /* each pair once, no global atomics (src/nbody_GPU_AOS_tiled.cu:44-220) */
-    NBODY_HIP_AUTO = -1      /* the library's choice for N */
+    NBODY_HIP_AUTO = -1      /* the autotuned plan, else SYMMETRIC for whole systems of N >= 1024, else LDS */
 };
 
-/* Options of the general entry; zero / -1 fields mean "choose". */
+/* Options of the general entry; zero / -1 fields mean "choose".
+ * For NBODY_HIP_SYMMETRIC, i_per_lane is 2, 4 or 8 and jslice is the block
+ * size B = 64 * i_per_lane * {1, 2, 4} (2 only with 4); its block slots are
+ * always summed with TwoSum, and it computes whole systems only (base 0, n N). */
 typedef struct nbody_hip_options {
     int variant;        /* NBODY_HIP_* or NBODY_HIP_AUTO */
-    int i_per_lane;     /* 1, 2, 4, or 0 = auto */
-    int jslice;         /* j bodies per slice, or 0 = auto */
+    int i_per_lane;     /* 1, 2, 4 (symmetric 2, 4, 8), or 0 = auto */
+    int jslice;         /* j bodies per slice (symmetric: block size), or 0 = auto */
     int compensated;    /* 1 TwoSum-compensated sums, 0 plain, -1 auto (on for N > 65536) */
     int newton;         /* 1 adds a Newton step after v_rsq_f32 */
 } nbody_hip_options;
@@ -72,7 +75,8 @@ float ComputeGravitation_HIP_SGPR(float *force, float *posMass, float softeningS
 /* Symmetric tiles: replaces ComputeGravitation_GPU_AOS_tiled (src/nbody_GPU_AOS_tiled.cu:196-220),
  * with the reaction scaled by m_i (the reference scales it by m_j). */
 float ComputeGravitation_HIP_Symmetric(float *force, float *posMass, float softeningSquared, size_t N);
-/* The library's fastest plan for N (autotuned once per size when enabled). */
+/* The library's choice for N: the plan nbody_hip_autotune found for this N
+ * if it ran, else the symmetric kernel (whole systems, N >= 1024) or LDS. */
 float ComputeGravitation_HIP(float *force, float *posMass, float softeningSquared, size_t N);
 
 /* Host-pointer twin with the ALGORITHM_AOS contract (src/nbody.cu:343-349): force and posMass are
@@ -117,9 +121,11 @@ int nbody_hip_energy_fp64(double *out, const float *posMass, const float *velInv
 /* The plan the library would use: fills variant, i_per_lane, jslice,
  * compensated, newton for an n-body i range of an N-body system. */
 int nbody_hip_plan(size_t N, size_t n, const nbody_hip_options *opt, nbody_hip_options *plan);
-/* Time candidate plans of `variant` on the given device data and keep the
- * fastest for this N (used by NBODY_HIP_AUTO from then on). Returns the best
- * kernel ms, 0.0f on failure. */
+/* Time candidate plans of `variant` (NBODY_HIP_AUTO: the LDS, SGPR and
+ * symmetric families) on the given device data, force kernel plus slice
+ * reduction, and keep the fastest for this N and variant; later calls with
+ * the same variant and default shape use it. Writes force. Returns the best
+ * ms, 0.0f on failure. */
 float nbody_hip_autotune(float *force, const float *posMass, float softeningSquared, size_t N, int variant);
 
 /* ---- measurement --------------------------------------------------------- */

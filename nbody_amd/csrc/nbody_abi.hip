@@ -86,7 +86,7 @@ struct DeviceState {
     Buffer partial;   // FP32 slice partials
     Buffer partial64; // FP64 slice partials / energy partials
     Buffer stage_pm, stage_force, small;
-    // tuned plans: (variant, N, n, comp, newton) -> plan
+    // tuned plans: (requested variant, N, n, requested compensation, newton) -> plan
     std::map<std::tuple<int, size_t, size_t, int, int>, ForcePlan> tuned;
     // kernel timer: event pairs recorded around each force kernel while enabled
     bool timing = false;
@@ -132,6 +132,15 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
     p.newton = opt && opt->newton > 0;
     const int comp_opt = opt ? opt->compensated : -1;
     p.comp = comp_opt < 0 ? (N > 65536) : comp_opt > 0;
+    if (variant == VAR_SYMMETRIC) {
+        // blocks of B = 64 * ipl * waves (default 8 bodies per lane, one wave:
+        // B = 512); the block slots are always summed with TwoSum
+        p.ipl = (opt && opt->i_per_lane > 0) ? opt->i_per_lane : 8;
+        p.jslice = (opt && opt->jslice > 0) ? opt->jslice : (p.ipl == 2 ? 512 : 64 * p.ipl);
+        p.nslices = (int)((N + p.jslice - 1) / p.jslice);
+        p.comp = true;
+        return p;
+    }
     if (variant == VAR_AOS) {
         p.ipl = 1;
         p.jslice = (int)N;
@@ -155,34 +164,59 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
     return p;
 }
 
-int resolve_variant(const nbody_hip_options* opt)
+// The untuned choice for NBODY_HIP_AUTO: the symmetric kernel for whole
+// systems of at least two of its blocks (half the pairs, 16 VALU + 1 rsq per
+// pair); otherwise the LDS broadcast, whose VALU operands all sit in VGPRs
+// (gfx950 issues those faster than ops reading an SGPR).
+int auto_variant(size_t N, size_t n) { return (n == N && N >= 1024) ? VAR_SYMMETRIC : VAR_LDS; }
+
+// Largest block count of a symmetric plan (the pair index must fit an int).
+constexpr int kMaxSymBlocks = 16384;
+
+std::tuple<int, size_t, size_t, int, int> tune_key(const nbody_hip_options* opt, size_t N, size_t n)
 {
-    int v = opt ? opt->variant : NBODY_HIP_AUTO;
-    if (v == NBODY_HIP_AUTO) v = VAR_SGPR;
-    return v;
+    const int v = opt ? opt->variant : NBODY_HIP_AUTO;
+    const int comp = opt ? opt->compensated : -1;
+    const int newton = opt && opt->newton > 0;
+    return std::make_tuple(v, N, n, comp, newton);
+}
+
+int validate_plan(const ForcePlan& p, size_t N)
+{
+    if (p.variant == VAR_SYMMETRIC) {
+        if (!symmetric_shape_ok(p.ipl, p.jslice)) {
+            set_error("symmetric plan needs i_per_lane 2/4/8 and a block of 64*i_per_lane*{1,2,4} bodies "
+                      "(got %d, %d)", p.ipl, p.jslice);
+            return E_ARG;
+        }
+        if (p.nslices > kMaxSymBlocks) {
+            set_error("N = %zu needs %d symmetric blocks of %d (at most %d)", N, p.nslices, p.jslice, kMaxSymBlocks);
+            return E_UNSUPPORTED;
+        }
+        return OK;
+    }
+    if (p.ipl != 1 && p.ipl != 2 && p.ipl != 4) {
+        set_error("i_per_lane must be 1, 2 or 4 (got %d)", p.ipl);
+        return E_ARG;
+    }
+    return OK;
 }
 
 int make_plan(DeviceState& d, size_t N, size_t n, const nbody_hip_options* opt, ForcePlan* plan)
 {
-    const int variant = resolve_variant(opt);
+    const int requested = opt ? opt->variant : NBODY_HIP_AUTO;
+    const int variant = requested == NBODY_HIP_AUTO ? auto_variant(N, n) : requested;
     if (variant < 0 || variant >= VAR_COUNT) {
-        set_error("unknown variant %d", variant);
-        return E_ARG;
-    }
-    if (variant == VAR_SYMMETRIC) {
-        set_error("the symmetric variant is not available in this build");
-        return E_UNSUPPORTED;
-    }
-    if (opt && opt->i_per_lane != 0 && opt->i_per_lane != 1 && opt->i_per_lane != 2 && opt->i_per_lane != 4) {
-        set_error("i_per_lane must be 1, 2 or 4 (got %d)", opt->i_per_lane);
+        set_error("unknown variant %d", requested);
         return E_ARG;
     }
     ForcePlan p = default_plan(variant, N, n, d.cus, opt);
     const bool explicit_shape = opt && (opt->i_per_lane > 0 || opt->jslice > 0);
     if (!explicit_shape) {
-        auto it = d.tuned.find(std::make_tuple(variant, N, n, (int)p.comp, (int)p.newton));
+        auto it = d.tuned.find(tune_key(opt, N, n));
         if (it != d.tuned.end()) p = it->second;
     }
+    if (int r = validate_plan(p, N)) return r;
     *plan = p;
     return OK;
 }
@@ -208,6 +242,10 @@ int check_args(const void* force, const void* pm, size_t N, size_t base, size_t 
 int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
               size_t n, hipStream_t stream)
 {
+    if (plan.variant == VAR_SYMMETRIC && (base != 0 || n != N)) {
+        set_error("the symmetric variant computes whole systems only (base = 0, n = N)");
+        return E_UNSUPPORTED;
+    }
     ForceParams p;
     p.pm = reinterpret_cast<const float4*>(pm);
     p.n = (int)N;
@@ -474,31 +512,46 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
     DeviceState* d;
     if (device_state(&d)) return 0.0f;
     nbody_hip_options base{variant, 0, 0, -1, 0};
-    ForcePlan p0;
-    if (make_plan(*d, N, N, &base, &p0)) return 0.0f;
-    if (variant == NBODY_HIP_AUTO) variant = p0.variant;
+    std::vector<int> families;
+    if (variant == NBODY_HIP_AUTO) families = {VAR_LDS, VAR_SGPR, VAR_SYMMETRIC};
+    else if (variant >= 0 && variant < VAR_COUNT) families = {variant};
+    else {
+        set_error("unknown variant %d", variant);
+        return 0.0f;
+    }
     std::vector<ForcePlan> cands;
-    if (p0.variant == VAR_AOS) {
-        cands.push_back(p0);
-    } else {
-        for (int ipl : {1, 2, 4}) {
-            nbody_hip_options o = base;
-            o.variant = p0.variant;
-            o.i_per_lane = ipl;
-            ForcePlan q = default_plan(p0.variant, N, N, d->cus, &o);
-            for (int mul : {1, 2, 4}) {
-                for (int div : {1, 2, 4}) {
-                    if (mul > 1 && div > 1) continue;
-                    ForcePlan c = q;
-                    c.jslice = std::min((int)N, round_up(std::max(256, q.jslice * mul / div), 256));
-                    c.nslices = (int)((N + c.jslice - 1) / c.jslice);
-                    cands.push_back(c);
+    for (int fam : families) {
+        nbody_hip_options o = base;
+        o.variant = fam;
+        if (fam == VAR_AOS) {
+            cands.push_back(default_plan(fam, N, N, d->cus, &o));
+        } else if (fam == VAR_SYMMETRIC) {
+            const int shapes[][2] = {{2, 512}, {4, 256}, {4, 512}, {4, 1024}, {8, 512}, {8, 1024}, {8, 2048}};
+            for (const auto& sh : shapes) {
+                o.i_per_lane = sh[0];
+                o.jslice = sh[1];
+                ForcePlan c = default_plan(fam, N, N, d->cus, &o);
+                if (c.nslices <= kMaxSymBlocks) cands.push_back(c);
+            }
+        } else {
+            for (int ipl : {1, 2, 4}) {
+                o.i_per_lane = ipl;
+                o.jslice = 0;
+                ForcePlan q = default_plan(fam, N, N, d->cus, &o);
+                for (int mul : {1, 2, 4}) {
+                    for (int div : {1, 2, 4}) {
+                        if (mul > 1 && div > 1) continue;
+                        ForcePlan c = q;
+                        c.jslice = std::min((int)N, round_up(std::max(256, q.jslice * mul / div), 256));
+                        c.nslices = (int)((N + c.jslice - 1) / c.jslice);
+                        cands.push_back(c);
+                    }
                 }
             }
         }
     }
     float best = 0.0f;
-    ForcePlan best_plan = p0;
+    ForcePlan best_plan = cands.front();
     for (const ForcePlan& c : cands) {
         const size_t bytes = plan_scratch_bytes(c, (int)N);
         if (bytes && d->partial.reserve(bytes) != hipSuccess) continue;
@@ -519,7 +572,7 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
             best_plan = c;
         }
     }
-    d->tuned[std::make_tuple(p0.variant, N, N, (int)p0.comp, (int)p0.newton)] = best_plan;
+    d->tuned[tune_key(&base, N, N)] = best_plan;
     return best;
 }
 

@@ -18,10 +18,12 @@ enum Variant : int {
 
 // One launch's decomposition: i bodies per lane, j range split into slices
 // whose partial sums are reduced in slice order (deterministic).
+// For VAR_SYMMETRIC, jslice is the block size B = 64 * ipl * waves (waves =
+// 1, 2 or 4 per workgroup) and nslices the number of blocks.
 struct ForcePlan {
     int variant = VAR_SGPR;
-    int ipl = 1;        // i bodies per lane: 1, 2 or 4
-    int jslice = 0;     // j bodies per slice
+    int ipl = 1;        // i bodies per lane: 1, 2, 4 (symmetric: 2, 4 or 8)
+    int jslice = 0;     // j bodies per slice (symmetric: bodies per block)
     int nslices = 1;    // ceil(n / jslice)
     bool comp = false;  // TwoSum-compensated accumulation
     bool newton = false;
@@ -39,6 +41,9 @@ struct ForceParams {
     float4* lo = nullptr;        // compensation partials [nslices][ni] (comp only)
     float* force = nullptr;      // 4 floats per i body; x, y, z written, w left alone
 };
+
+// Whether (i per lane, block size) is a compiled symmetric shape.
+bool symmetric_shape_ok(int ipl, int block);
 
 // Bytes of slice-partial scratch a plan needs for ni bodies (0 when direct).
 size_t plan_scratch_bytes(const ForcePlan& plan, int ni);
