@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--compensated", type=int, default=-1, choices=[-1, 0, 1])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration for --pmc runs)")
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble"],
+                    help="single: one N-body system (the headline line); ensemble: 512 independent systems")
+    ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=15.0)
     return ap.parse_args()
@@ -130,6 +133,59 @@ def traffic_from_profiles(workload_key):
         return None
 
 
+def bench_ensemble(args):
+    """512 independent uniform balls, N log-uniform in [256, 16384] (seeded),
+    eps^2 = 0.01, dt = 0.016, 50 steps: one persistent force launch + one
+    integrate launch per step replayed from a graph, against every system
+    run on its own with its best single-system plan."""
+    import numpy as np
+    from nbody_amd import _native, ics
+    L = _native.lib()
+    steps = args.steps if args.steps != 20 else 50
+    sizes = ics.log_uniform_sizes(args.systems, 256, 16384, seed=4)
+    systems = [ics.uniform_ball(int(n), seed=100000 + k) for k, n in enumerate(sizes)]
+    pm = np.ascontiguousarray(np.concatenate([s[0] for s in systems]))
+    vm = np.ascontiguousarray(np.concatenate([s[1] for s in systems]))
+    nsz = np.ascontiguousarray(sizes.astype(np.uint64))
+    e2 = np.full(len(systems), 0.01, np.float32)
+    inter = float((sizes.astype(np.float64) ** 2).sum())
+    res = {}
+    for name, mode in (("batched_graph", _native.BATCHED_GRAPH), ("batched", _native.BATCHED),
+                       ("per_system", _native.PER_SYSTEM)):
+        h = L.nbody_hip_ensemble_create(len(systems), nsz.ctypes.data, pm.ctypes.data, vm.ctypes.data,
+                                        e2.ctypes.data, 0.016, 0.95)
+        if not h:
+            raise _native.NativeError(_native.last_error())
+        try:
+            ms = ctypes.c_float()
+            _native.check(L.nbody_hip_ensemble_run(h, max(args.warmup // 10, 2), mode, ctypes.byref(ms)), name)
+            _native.check(L.nbody_hip_ensemble_run(h, steps, mode, ctypes.byref(ms)), name)
+            res[name] = {"ms": ms.value, "value": inter * steps / (ms.value * 1e-3) / 1e9}
+        finally:
+            L.nbody_hip_ensemble_destroy(h)
+    v = res["batched_graph"]["value"]
+    out = {
+        "metric": METRIC,
+        "value": round(v, 2),
+        "unit": "10^9 interactions/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "ms_per_step": round(res["batched_graph"]["ms"] / steps, 4),
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform balls, unit masses)",
+        "config": {"workload": f"ensemble of {len(systems)} independent systems, N log-uniform in [256, 16384] "
+                               f"(seed 4; total {int(sizes.sum())} bodies, sum N^2 = {inter:.3e}), eps^2=0.01, "
+                               f"dt=0.016, damping=0.95, {steps} steps",
+                   "systems": len(systems)},
+        "modes": {k: {"ms_per_step": round(r["ms"] / steps, 4), "value": round(r["value"], 2)}
+                  for k, r in res.items()},
+        "speedup_vs_per_system": round(v / res["per_system"]["value"], 2),
+        "achieved_tflops": round(20 * v / 1e3, 2),
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -153,6 +209,9 @@ def main():
         print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
                           "launches": 5}))
         return
+
+    if args.workload == "ensemble":
+        return bench_ensemble(args)
 
     L = _native.lib()
     n, K, W = args.n, args.steps, args.warmup

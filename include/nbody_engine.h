@@ -1,0 +1,75 @@
+/*
+ * nbody_engine.h — device-resident stepping on top of the force entries of
+ * nbody_hip.h: one system, or an ensemble of independent systems.
+ *
+ * The reference's step (src/nbody.cu:293-470, driven by src/nbody.cu:1015-1088)
+ * copies positions to the device, runs the force entry, copies forces back and
+ * integrates on the host (src/nbody.cu:352-370,420-441). Here positions,
+ * velocities and forces stay in HBM; a step is the force evaluation followed by
+ * integrateGravitation (src/nbody.cu:158-236), queued on the object's own
+ * stream, optionally replayed from a captured HIP graph.
+ *
+ * All functions return 0 on success or a nonzero code with the message in
+ * nbody_hip_last_error(); create functions return NULL on failure.
+ * Buffers are host float4 arrays (posMass x,y,z,m; velInvMass vx,vy,vz,1/m).
+ */
+#ifndef NBODY_ENGINE_H
+#define NBODY_ENGINE_H
+
+#include <stddef.h>
+
+#include "nbody_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- one system ------------------------------------------------------------ */
+
+typedef struct nbody_hip_system nbody_hip_system;
+
+/* Uploads N bodies and fixes the plan (opt as for nbody_hip_force; NULL: the
+ * library's choice, including a plan nbody_hip_autotune found for this N). */
+nbody_hip_system *nbody_hip_system_create(const float *posMass, const float *velInvMass, size_t N,
+                                          float softeningSquared, float dt, float damping,
+                                          const nbody_hip_options *opt);
+/* Runs `steps` steps and waits for them; *ms (may be NULL) receives their
+ * elapsed time measured with events on the system's stream. use_graph = 1
+ * replays one captured graph per step (force kernel, slice/slot reduction,
+ * integrator; captured on first use). */
+int nbody_hip_system_run(nbody_hip_system *s, int steps, int use_graph, float *ms);
+/* Copies the current state (and the last force) to the host; NULL skips one. */
+int nbody_hip_system_read(const nbody_hip_system *s, float *posMass, float *velInvMass, float *force);
+/* The plan in use. */
+int nbody_hip_system_plan(const nbody_hip_system *s, nbody_hip_options *plan);
+void nbody_hip_system_destroy(nbody_hip_system *s);
+
+/* ---- ensembles of independent systems ------------------------------------ */
+
+typedef struct nbody_hip_ensemble nbody_hip_ensemble;
+
+enum {
+    NBODY_HIP_BATCHED_GRAPH = 0, /* one persistent force launch + one integrate launch per step, from a graph */
+    NBODY_HIP_BATCHED = 1,       /* the same two launches per step, without a graph */
+    NBODY_HIP_PER_SYSTEM = 2     /* every system on its own: its best single-system plan, one after another */
+};
+
+/* count systems of N[s] bodies, stored back to back in posMass/velInvMass
+ * (system s starts at body sum_{r<s} N[r]); softeningSquared[s] per system;
+ * dt and damping shared. */
+nbody_hip_ensemble *nbody_hip_ensemble_create(size_t count, const size_t *N, const float *posMass,
+                                              const float *velInvMass, const float *softeningSquared, float dt,
+                                              float damping);
+/* Runs `steps` steps of every system in `mode` and waits; *ms as above. In
+ * the batched modes each body's force comes from the same arithmetic
+ * whatever the other systems are, so a system gives bit-identical results
+ * alone and in any ensemble. */
+int nbody_hip_ensemble_run(nbody_hip_ensemble *e, int steps, int mode, float *ms);
+int nbody_hip_ensemble_read(const nbody_hip_ensemble *e, float *posMass, float *velInvMass, float *force);
+void nbody_hip_ensemble_destroy(nbody_hip_ensemble *e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBODY_ENGINE_H */

@@ -11,6 +11,9 @@ LIB_PATH = os.path.join(HERE, "lib", "libnbody_hip.so")
 
 ABI_VERSION = 1
 
+# ensemble modes (include/nbody_engine.h)
+BATCHED_GRAPH, BATCHED, PER_SYSTEM = 0, 1, 2
+
 # variant ids (NBODY_HIP_* in include/nbody_hip.h)
 AOS, LDS, DPP, SGPR, SYMMETRIC, AUTO = 0, 1, 2, 3, 4, -1
 VARIANT_NAMES = {AOS: "AOS", LDS: "LDS", DPP: "DPP", SGPR: "SGPR", SYMMETRIC: "Symmetric", AUTO: "auto"}
@@ -56,6 +59,15 @@ SIGNATURES = {
     "nbody_hip_last_error": (ctypes.c_char_p, []),
     "nbody_hip_abi_version": (_i, []),
     "nbody_hip_device_info": (_i, [ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.c_char_p, _sz]),
+    "nbody_hip_system_create": (_p, [_p, _p, _sz, _f, _f, _f, _OPT]),
+    "nbody_hip_system_run": (_i, [_p, _i, _i, ctypes.POINTER(_f)]),
+    "nbody_hip_system_read": (_i, [_p, _p, _p, _p]),
+    "nbody_hip_system_plan": (_i, [_p, _OPT]),
+    "nbody_hip_system_destroy": (None, [_p]),
+    "nbody_hip_ensemble_create": (_p, [_sz, _p, _p, _p, _p, _f, _f]),
+    "nbody_hip_ensemble_run": (_i, [_p, _i, _i, ctypes.POINTER(_f)]),
+    "nbody_hip_ensemble_read": (_i, [_p, _p, _p, _p]),
+    "nbody_hip_ensemble_destroy": (None, [_p]),
     "nbody_random_unit_bodies": (_i, [_p, _p, _sz, _u32, _f, _f]),
     "nbody_uniform_ball": (_i, [_p, _p, _sz, _u32, _f, _f, _f]),
     "nbody_plummer": (_i, [_p, _p, _sz, _u32, _f, _f, _i]),

@@ -24,7 +24,7 @@ INCLUDE = os.path.join(ROOT, "include")
 
 ARCH = os.environ.get("NBODY_HIP_ARCH", "gfx950")
 
-HIP_SOURCES = ["nbody_kernels.hip", "nbody_abi.hip"]
+HIP_SOURCES = ["nbody_kernels.hip", "nbody_batch.hip", "nbody_abi.hip", "nbody_engine.hip"]
 # host-only C++, strict IEEE (its arithmetic is written out explicitly)
 CXX_SOURCES = ["nbody_ics.cpp"]
 CXX_FLAGS = ["-O2", "-fno-fast-math", "-ffp-contract=off", "-std=c++17", "-fPIC", "-Wall", "-I", INCLUDE]

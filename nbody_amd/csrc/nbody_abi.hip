@@ -3,8 +3,7 @@
 // Error convention of the reference (src/chError.h:86-91 + src/nbody.cu:385-386):
 // a failing table entry returns 0.0f and the driver skips it. Here the reason is
 // also kept per thread for nbody_hip_last_error().
-#include "nbody_hip.h"
-#include "nbody_kernels.h"
+#include "nbody_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -22,9 +21,13 @@
 
 using namespace nbody;
 
-namespace {
+namespace nbody {
+namespace detail {
 
 thread_local std::string g_last_error;
+std::mutex g_mutex;
+std::vector<DeviceState> g_devices;
+
 
 void set_error(const char* fmt, ...)
 {
@@ -35,70 +38,6 @@ void set_error(const char* fmt, ...)
     va_end(ap);
     g_last_error = buf;
 }
-
-enum : int { OK = 0, E_ARG = 1, E_HIP = 2, E_UNSUPPORTED = 3 };
-
-#define HIP_TRY(expr)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess) {                                                        \
-            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
-                      __LINE__);                                                       \
-            return E_HIP;                                                              \
-        }                                                                              \
-    } while (0)
-
-// Largest N accepted: float4 offsets and 4*N fit comfortably in int.
-constexpr size_t kMaxBodies = size_t(1) << 28;
-
-// Device buffer that only grows. Growing waits for the device first, so a
-// buffer still read by queued kernels is never freed under them.
-struct Buffer {
-    void* ptr = nullptr;
-    size_t cap = 0;
-    hipError_t reserve(size_t bytes)
-    {
-        if (bytes <= cap) return hipSuccess;
-        if (ptr) {
-            hipError_t e = hipDeviceSynchronize();
-            if (e != hipSuccess) return e;
-            (void)hipFree(ptr);
-            ptr = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max(bytes, cap + cap / 2);
-        hipError_t e = hipMalloc(&ptr, want);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            return e;
-        }
-        cap = want;
-        return hipSuccess;
-    }
-};
-
-struct DeviceState {
-    bool init = false;
-    int cus = 0;
-    int clock_khz = 0;
-    char arch[64] = {0};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    Buffer partial;   // FP32 slice partials
-    Buffer partial64; // FP64 slice partials / energy partials
-    Buffer stage_pm, stage_force, small;
-    // tuned plans: (requested variant, N, n, requested compensation, newton) -> plan
-    std::map<std::tuple<int, size_t, size_t, int, int>, ForcePlan> tuned;
-    // kernel timer: event pairs recorded around each force kernel while enabled
-    bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> timer_events;
-    size_t timer_used = 0;
-};
-
-// Largest number of force launches the kernel timer records between reads.
-constexpr size_t kMaxTimedLaunches = 1 << 14;
-
-std::mutex g_mutex;
-std::vector<DeviceState> g_devices;
 
 int device_state(DeviceState** out)
 {
@@ -240,8 +179,9 @@ int check_args(const void* force, const void* pm, size_t N, size_t base, size_t 
 
 // Queue one force evaluation on `stream` (caller holds g_mutex).
 int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
-              size_t n, hipStream_t stream)
+              size_t n, hipStream_t stream, Buffer* scratch, bool timed)
 {
+    Buffer& buf = scratch ? *scratch : d.partial;
     if (plan.variant == VAR_SYMMETRIC && (base != 0 || n != N)) {
         set_error("the symmetric variant computes whole systems only (base = 0, n = N)");
         return E_UNSUPPORTED;
@@ -257,12 +197,12 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
     p.force = force;
     const size_t bytes = plan_scratch_bytes(plan, (int)n);
     if (bytes) {
-        HIP_TRY(d.partial.reserve(bytes));
-        p.hi = static_cast<float4*>(d.partial.ptr);
+        HIP_TRY(buf.reserve(bytes));
+        p.hi = static_cast<float4*>(buf.ptr);
         p.lo = plan.comp ? p.hi + (size_t)plan.nslices * n : nullptr;
     }
     hipEvent_t before = nullptr, after = nullptr;
-    if (d.timing && d.timer_used < kMaxTimedLaunches) {
+    if (timed && d.timing && d.timer_used < kMaxTimedLaunches) {
         if (d.timer_used == d.timer_events.size()) {
             hipEvent_t a, b;
             HIP_TRY(hipEventCreate(&a));
@@ -276,6 +216,13 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
     HIP_TRY(launch_force(plan, p, stream, before, after));
     return OK;
 }
+
+}  // namespace detail
+}  // namespace nbody
+
+using namespace nbody::detail;
+
+namespace {
 
 // Synchronous, event-timed evaluation on the null stream; returns ms or 0.
 float timed_force(const nbody_hip_options* opt, float* force, const float* pm, float eps2, size_t N)

@@ -67,4 +67,51 @@ __device__ __forceinline__ float row_bcast(float v)
                                                                  0x150 + K, 0xf, 0xf, false));
 }
 
+// Per-lane state of I i bodies (or their accumulators).
+template <int I>
+struct Lanes {
+    float x[I], y[I], z[I];
+};
+
+template <int I>
+__device__ __forceinline__ void zero(Lanes<I>& a)
+{
+#pragma unroll
+    for (int u = 0; u < I; ++u) a.x[u] = a.y[u] = a.z[u] = 0.f;
+}
+
+// hi += part with TwoSum, residue into lo.
+template <int I>
+__device__ __forceinline__ void fold(Lanes<I>& hi, Lanes<I>& lo, const Lanes<I>& part)
+{
+#pragma unroll
+    for (int u = 0; u < I; ++u) {
+        float e;
+        hi.x[u] = two_sum(hi.x[u], part.x[u], e);
+        lo.x[u] += e;
+        hi.y[u] = two_sum(hi.y[u], part.y[u], e);
+        lo.y[u] += e;
+        hi.z[u] = two_sum(hi.z[u], part.z[u], e);
+        lo.z[u] += e;
+    }
+}
+
+// j bodies read with wave-uniform addresses: the compiler emits s_load_dwordx16
+// (four bodies per load) and the VALU ops take x_j, y_j, z_j, m_j as SGPR
+// operands, so no instruction is spent on the broadcast; on gfx950 a VALU op
+// that reads an SGPR issues at about 60% of the all-VGPR rate, though.
+template <int I, bool NEWTON>
+__device__ __forceinline__ void sweep_sgpr(const float4* __restrict__ src, int len, float eps2,
+                                           const Lanes<I>& me, Lanes<I>& acc)
+{
+#pragma unroll 8
+    for (int j = 0; j < len; ++j) {
+        const float4 b = src[j];
+#pragma unroll
+        for (int u = 0; u < I; ++u)
+            body_body<NEWTON>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
+                              acc.z[u]);
+    }
+}
+
 }  // namespace nbody

@@ -70,3 +70,36 @@ hipError_t launch_energy_fp64(double* acc, const float4* pm, const float4* vim, 
                               double* partial, hipStream_t stream);
 
 }  // namespace nbody
+
+namespace nbody {
+
+// ---- batches of independent systems (ensembles, served requests) ----------
+
+// One unit of the batched work queue: the force on bodies [i0, i0 + ni)
+// (ni <= 256 * ipl) of a system whose bodies are [j0, j0 + nj) in the
+// concatenated arrays. 32 bytes.
+struct BatchTask {
+    int i0;
+    int ni;
+    int j0;
+    int nj;
+    float eps2;
+    int pad[3];
+};
+
+// Largest ni a task may have for a given ipl.
+constexpr int batch_task_bodies(int ipl) { return 256 * ipl; }
+
+// Persistent force kernel: `blocks` workgroups take tasks from `counter`
+// (which must read 0 at launch) until all ntasks are done. Each body's force
+// depends only on its own system: TwoSum-folded 128-body chunks, the same
+// arithmetic whatever else is in the batch.
+hipError_t launch_force_batch(const float4* pm, float* force, const BatchTask* tasks, int ntasks, int ipl,
+                              unsigned* counter, int blocks, hipStream_t stream);
+
+// Integrate the bodies of every task (dt, damping as integrateGravitation)
+// and reset `counter` to 0 for the next force launch.
+hipError_t launch_integrate_batch(float4* pm, float4* vim, const float4* force, const BatchTask* tasks, int ntasks,
+                                  float dt, float damping, unsigned* counter, hipStream_t stream);
+
+}  // namespace nbody

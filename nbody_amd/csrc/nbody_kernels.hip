@@ -23,11 +23,6 @@ constexpr int kBlock = 256;
 constexpr int kChunk = 128;  // j bodies between TwoSum folds (multiple of 16)
 
 template <int I>
-struct Lanes {
-    float x[I], y[I], z[I];
-};
-
-template <int I>
 __device__ __forceinline__ void load_i(const ForceParams& p, Lanes<I>& me)
 {
     const int mbase = blockIdx.x * (kBlock * I) + threadIdx.x;
@@ -38,29 +33,6 @@ __device__ __forceinline__ void load_i(const ForceParams& p, Lanes<I>& me)
         me.x[u] = q.x;
         me.y[u] = q.y;
         me.z[u] = q.z;
-    }
-}
-
-template <int I>
-__device__ __forceinline__ void zero(Lanes<I>& a)
-{
-#pragma unroll
-    for (int u = 0; u < I; ++u) a.x[u] = a.y[u] = a.z[u] = 0.f;
-}
-
-// hi += part with TwoSum, residue into lo.
-template <int I>
-__device__ __forceinline__ void fold(Lanes<I>& hi, Lanes<I>& lo, const Lanes<I>& part)
-{
-#pragma unroll
-    for (int u = 0; u < I; ++u) {
-        float e;
-        hi.x[u] = two_sum(hi.x[u], part.x[u], e);
-        lo.x[u] += e;
-        hi.y[u] = two_sum(hi.y[u], part.y[u], e);
-        lo.y[u] += e;
-        hi.z[u] = two_sum(hi.z[u], part.z[u], e);
-        lo.z[u] += e;
     }
 }
 
@@ -88,23 +60,6 @@ __device__ __forceinline__ void store(const ForceParams& p, const Lanes<I>& hi, 
             p.hi[k] = make_float4(hi.x[u], hi.y[u], hi.z[u], 0.f);
             if (lo) p.lo[k] = make_float4(lo->x[u], lo->y[u], lo->z[u], 0.f);
         }
-    }
-}
-
-// j bodies read with wave-uniform addresses: the compiler emits s_load_dwordx16
-// (four bodies per load) and every VALU op takes x_j, y_j, z_j, m_j as SGPR
-// operands, so no VALU or LDS work is spent on the broadcast.
-template <int I, bool NEWTON>
-__device__ __forceinline__ void sweep_sgpr(const float4* __restrict__ src, int len, float eps2,
-                                           const Lanes<I>& me, Lanes<I>& acc)
-{
-#pragma unroll 8
-    for (int j = 0; j < len; ++j) {
-        const float4 b = src[j];
-#pragma unroll
-        for (int u = 0; u < I; ++u)
-            body_body<NEWTON>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
-                              acc.z[u]);
     }
 }
 

@@ -1,0 +1,193 @@
+"""Device-resident stepping (include/nbody_engine.h) on the GPU.
+
+- graph replays against eager launches, bit for bit, with eager work and
+  synchronisations between the replays;
+- the reference's per-step crosscheck (src/nbody.cu:313-325,406-418): at every
+  step the GPU force on the GPU's current positions against the oracle's
+  CPU_AOS on the same positions, and the GPU integrator against the oracle's
+  integrator given the same force (bit-exact);
+- ensembles: batched (graph and plain) against one system at a time, each
+  system alone against the same system in a mix (bit-exact), forces against
+  the FP64 oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import EPS2
+from oracle_lib import force_errors
+
+from nbody_amd import _native, ics
+
+pytestmark = pytest.mark.gpu
+L = None
+
+
+def lib():
+    global L
+    if L is None:
+        L = _native.lib()
+    return L
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+class System:
+    def __init__(self, pm, vm, eps2, dt=0.016, damping=0.95, **opt):
+        self.n = pm.shape[0]
+        o = _native.Options.make(**opt) if opt else None
+        self.h = lib().nbody_hip_system_create(pm.ctypes.data, vm.ctypes.data, self.n, eps2, dt, damping,
+                                               ctypes.byref(o) if o else None)
+        assert self.h, _native.last_error()
+
+    def run(self, steps, graph):
+        ms = ctypes.c_float()
+        _native.check(lib().nbody_hip_system_run(self.h, steps, 1 if graph else 0, ctypes.byref(ms)), "run")
+        return ms.value
+
+    def read(self):
+        pm = np.zeros((self.n, 4), np.float32)
+        vm = np.zeros((self.n, 4), np.float32)
+        f = np.zeros((self.n, 4), np.float32)
+        _native.check(lib().nbody_hip_system_read(self.h, pm.ctypes.data, vm.ctypes.data, f.ctypes.data), "read")
+        return pm, vm, f
+
+    def close(self):
+        lib().nbody_hip_system_destroy(self.h)
+
+
+class Ensemble:
+    def __init__(self, systems, eps2, dt=0.016, damping=0.95):
+        self.sizes = np.array([s[0].shape[0] for s in systems], np.uint64)
+        pm = np.ascontiguousarray(np.concatenate([s[0] for s in systems]))
+        vm = np.ascontiguousarray(np.concatenate([s[1] for s in systems]))
+        e2 = np.full(len(systems), eps2, np.float32)
+        self.total = pm.shape[0]
+        self.h = lib().nbody_hip_ensemble_create(len(systems), self.sizes.ctypes.data, pm.ctypes.data, vm.ctypes.data,
+                                                 e2.ctypes.data, dt, damping)
+        assert self.h, _native.last_error()
+
+    def run(self, steps, mode):
+        ms = ctypes.c_float()
+        _native.check(lib().nbody_hip_ensemble_run(self.h, steps, mode, ctypes.byref(ms)), "ensemble run")
+        return ms.value
+
+    def read(self):
+        pm = np.zeros((self.total, 4), np.float32)
+        vm = np.zeros((self.total, 4), np.float32)
+        f = np.zeros((self.total, 4), np.float32)
+        _native.check(lib().nbody_hip_ensemble_read(self.h, pm.ctypes.data, vm.ctypes.data, f.ctypes.data), "read")
+        return pm, vm, f
+
+    def split(self, a):
+        return np.split(a, np.cumsum(self.sizes.astype(np.int64))[:-1])
+
+    def close(self):
+        lib().nbody_hip_ensemble_destroy(self.h)
+
+
+@pytest.mark.parametrize("opt", [{}, {"variant": _native.LDS}, {"variant": _native.SGPR, "i_per_lane": 2}])
+def test_graph_replays_match_eager(opt):
+    pm, vm = ics.uniform_ball(6000, seed=3)
+    a = System(pm, vm, 0.01, **opt)
+    b = System(pm, vm, 0.01, **opt)
+    try:
+        for k in range(4):
+            a.run(2, graph=True)         # replay 2 steps from the graph
+            b.run(2, graph=False)        # the same two steps launched one by one
+            sa, sb = a.read(), b.read()  # host synchronisation between replays
+            for x, y in zip(sa, sb):
+                assert np.array_equal(bits(x), bits(y)), (opt, k)
+            _ = np.linalg.norm(sa[0])    # eager host work in between
+    finally:
+        a.close()
+        b.close()
+
+
+def test_step_crosscheck_every_step(oracle):
+    """Ten steps of the N = 16384 uniform-ball workload, crosschecked at every
+    step on identical positions against the oracle's CPU_AOS (the reference's
+    crosscheck target), and the integrator bit-exact against the oracle's."""
+    n = 16384
+    pm0, vm0 = ics.uniform_ball(n, seed=1)
+    s = System(pm0, vm0, 0.01)
+    prev_pm, prev_vm = pm0, vm0
+    try:
+        for step in range(10):
+            s.run(1, graph=True)
+            pm, vm, f = s.read()
+            aos, _ = oracle.cpu_aos(prev_pm, np.float32(0.01))
+            f64 = oracle.fp64_forces(prev_pm, np.float32(0.01))
+            d = np.linalg.norm(f[:, :3].astype(np.float64) - aos[:, :3], axis=1) / np.linalg.norm(f64, axis=1)
+            mx, rms = force_errors(f, f64)
+            assert d.max() <= 2e-5 and mx <= 1e-5 and rms <= 1e-6, (step, d.max(), mx, rms)
+            p_ref, v_ref = oracle.integrate_aos(prev_pm, prev_vm, f, 0.016, 0.95)
+            assert np.array_equal(bits(pm), bits(p_ref)) and np.array_equal(bits(vm), bits(v_ref)), step
+            prev_pm, prev_vm = pm, vm
+    finally:
+        s.close()
+
+
+def make_ensemble_systems(count, lo, hi, seed):
+    sizes = ics.log_uniform_sizes(count, lo, hi, seed)
+    return [ics.uniform_ball(int(n), seed=seed * 1000 + k) for k, n in enumerate(sizes)]
+
+
+def test_ensemble_modes_and_forces(oracle):
+    systems = make_ensemble_systems(24, 17, 3000, seed=5)
+    e_graph = Ensemble(systems, EPS2)
+    e_plain = Ensemble(systems, EPS2)
+    e_each = Ensemble(systems, EPS2)
+    try:
+        e_graph.run(1, _native.BATCHED_GRAPH)
+        e_plain.run(1, _native.BATCHED)
+        e_each.run(1, _native.PER_SYSTEM)
+        g, p, q = e_graph.read(), e_plain.read(), e_each.read()
+        for x, y in zip(g, p):
+            assert np.array_equal(bits(x), bits(y))
+        # forces of step 1 are the forces of the initial positions
+        for (pm, _), fg, fq in zip(systems, e_graph.split(g[2]), e_each.split(q[2])):
+            f64 = oracle.fp64_forces(pm, EPS2)
+            if pm.shape[0] == 1:
+                continue
+            for f in (fg, fq):
+                mx, rms = force_errors(f, f64)
+                assert mx <= 1e-5 and rms <= 1e-6, (pm.shape[0], mx, rms)
+        for k in range(3):  # several graph replays with host work between
+            e_graph.run(2, _native.BATCHED_GRAPH)
+            e_plain.run(2, _native.BATCHED)
+            for x, y in zip(e_graph.read(), e_plain.read()):
+                assert np.array_equal(bits(x), bits(y)), k
+    finally:
+        for e in (e_graph, e_plain, e_each):
+            e.close()
+
+
+def test_ensemble_alone_equals_in_mix():
+    systems = make_ensemble_systems(40, 256, 5000, seed=6)
+    mix = Ensemble(systems, EPS2)
+    try:
+        mix.run(3, _native.BATCHED_GRAPH)
+        got = [mix.split(a) for a in mix.read()]
+        for k in (0, 7, 39):
+            alone = Ensemble([systems[k]], EPS2)
+            try:
+                alone.run(3, _native.BATCHED_GRAPH)
+                for a, m in zip(alone.read(), got):
+                    assert np.array_equal(bits(a), bits(m[k])), k
+            finally:
+                alone.close()
+    finally:
+        mix.close()
+
+
+def test_ensemble_rejects_bad_input():
+    n = np.array([0], np.uint64)
+    pm = np.zeros((1, 4), np.float32)
+    e2 = np.zeros(1, np.float32)
+    h = lib().nbody_hip_ensemble_create(1, n.ctypes.data, pm.ctypes.data, pm.ctypes.data, e2.ctypes.data, 0.016,
+                                        0.95)
+    assert not h and "N = 0" in _native.last_error()
