@@ -35,6 +35,36 @@ class Options(ctypes.Structure):
         return cls(variant, i_per_lane, jslice, compensated, newton)
 
 
+class Request(ctypes.Structure):
+    """nbody_hip_request (include/nbody_serve.h)."""
+
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("N", ctypes.c_size_t),
+        ("steps", ctypes.c_int),
+        ("seed", ctypes.c_uint32),
+        ("ic", ctypes.c_int),
+        ("softeningSquared", ctypes.c_float),
+        ("arrival_s", ctypes.c_double),
+    ]
+
+
+class Result(ctypes.Structure):
+    """nbody_hip_result (include/nbody_serve.h)."""
+
+    _fields_ = [
+        ("id", ctypes.c_uint64),
+        ("N", ctypes.c_size_t),
+        ("steps", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("arrival_s", ctypes.c_double),
+        ("admit_s", ctypes.c_double),
+        ("finish_s", ctypes.c_double),
+    ]
+
+
+IC_UNIFORM_BALL, IC_PLUMMER = 0, 1
+
 # C signatures: name -> (restype, argtypes)
 _f, _d, _i, _sz, _p, _u32 = (ctypes.c_float, ctypes.c_double, ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p,
                              ctypes.c_uint32)
@@ -68,6 +98,12 @@ SIGNATURES = {
     "nbody_hip_ensemble_run": (_i, [_p, _i, _i, ctypes.POINTER(_f)]),
     "nbody_hip_ensemble_read": (_i, [_p, _p, _p, _p]),
     "nbody_hip_ensemble_destroy": (None, [_p]),
+    "nbody_hip_server_create": (_p, [_sz, _i, _f, _f, _i]),
+    "nbody_hip_server_run_trace": (_i, [_p, ctypes.POINTER(Request), _sz, ctypes.POINTER(Result),
+                                        ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+    "nbody_hip_server_final_state": (_i, [_p, ctypes.c_uint64, _p, _p]),
+    "nbody_hip_server_peak_active": (_i, [_p]),
+    "nbody_hip_server_destroy": (None, [_p]),
     "nbody_random_unit_bodies": (_i, [_p, _p, _sz, _u32, _f, _f]),
     "nbody_uniform_ball": (_i, [_p, _p, _sz, _u32, _f, _f, _f]),
     "nbody_plummer": (_i, [_p, _p, _sz, _u32, _f, _f, _i]),

@@ -114,4 +114,24 @@ __device__ __forceinline__ void sweep_sgpr(const float4* __restrict__ src, int l
     }
 }
 
+// Read-only view of body records for kernels whose body pointers are not
+// kernel arguments (batched tasks carry them in memory): the constant address
+// space tells the compiler no lane writes them during the launch, so the
+// wave-uniform reads stay scalar loads (s_load) into SGPR operands.
+typedef const __attribute__((address_space(4))) float const_body_float;
+
+template <int I, bool NEWTON>
+__device__ __forceinline__ void sweep_const(const float4* src4, int len, float eps2, const Lanes<I>& me,
+                                            Lanes<I>& acc)
+{
+    const const_body_float* src = (const const_body_float*)(const void*)src4;
+#pragma unroll 8
+    for (int j = 0; j < len; ++j) {
+        const float xj = src[4 * j + 0], yj = src[4 * j + 1], zj = src[4 * j + 2], mj = src[4 * j + 3];
+#pragma unroll
+        for (int u = 0; u < I; ++u)
+            body_body<NEWTON>(xj, yj, zj, mj, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u], acc.z[u]);
+    }
+}
+
 }  // namespace nbody

@@ -1,14 +1,22 @@
 // Device-resident stepping (include/nbody_engine.h): one system with a
 // captured per-step graph, and ensembles of independent systems run either
 // batched (a persistent work-queue force launch over all of them) or one
-// system after another.
+// system after another; and the request server (include/nbody_serve.h), which
+// runs the same batched launch over a changing set of resident requests.
 #include "nbody_engine.h"
+#include "nbody_ics.h"
 #include "nbody_internal.h"
+#include "nbody_serve.h"
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 using namespace nbody;
@@ -16,11 +24,32 @@ using namespace nbody::detail;
 
 namespace {
 
-// Workgroups of the persistent batch kernel per CU: its ~50 VGPRs allow 8
-// waves per SIMD, i.e. 8 four-wave workgroups per CU.
-constexpr int kBatchBlocksPerCU = 8;
-// i bodies per lane in a batch task (256 per task).
-constexpr int kBatchIpl = 1;
+// One-wave workgroups of the persistent batch kernel per CU: its ~50 VGPRs
+// allow 8 waves per SIMD, 32 per CU.
+constexpr int kBatchBlocksPerCU = 32;
+
+// The (system, 64-body block) tasks of one system.
+void append_tasks(std::vector<BatchTask>& out, float4* pm, float4* vm, float4* force, size_t n, float eps2)
+{
+    for (size_t b = 0; b < n; b += kBatchTaskBodies) {
+        BatchTask t{};
+        t.pm = pm;
+        t.vm = vm;
+        t.force = force;
+        t.i0 = (int)b;
+        t.ni = (int)std::min<size_t>(kBatchTaskBodies, n - b);
+        t.n = (int)n;
+        t.eps2 = eps2;
+        out.push_back(t);
+    }
+}
+
+// Largest systems first, so the tail of the persistent launch holds the
+// cheapest tasks.
+void order_tasks(std::vector<BatchTask>& tasks)
+{
+    std::stable_sort(tasks.begin(), tasks.end(), [](const BatchTask& a, const BatchTask& b) { return a.n > b.n; });
+}
 
 template <class T>
 hipError_t dev_alloc(T** p, size_t count)
@@ -123,9 +152,8 @@ struct nbody_hip_ensemble {
 
     int queue_batched()
     {
-        HIP_TRY(launch_force_batch(pm, reinterpret_cast<float*>(force), tasks, ntasks, kBatchIpl, counter, blocks,
-                                   stream));
-        HIP_TRY(launch_integrate_batch(pm, vm, force, tasks, ntasks, dt, damping, counter, stream));
+        HIP_TRY(launch_force_batch(tasks, ntasks, counter, blocks, stream));
+        HIP_TRY(launch_integrate_batch(tasks, ntasks, dt, damping, counter, stream));
         return OK;
     }
 
@@ -288,22 +316,9 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
         delete e;
         return nullptr;
     }
-    // (system, 256-body i block) tasks, largest systems first so the tail of
-    // the persistent launch is made of the cheapest tasks
-    std::vector<BatchTask> tasks;
-    const int per = batch_task_bodies(kBatchIpl);
-    for (size_t s = 0; s < count; ++s)
-        for (size_t b = 0; b < N[s]; b += per) {
-            BatchTask t{};
-            t.i0 = (int)(e->off[s] + b);
-            t.ni = (int)std::min<size_t>(per, N[s] - b);
-            t.j0 = (int)e->off[s];
-            t.nj = (int)N[s];
-            t.eps2 = softeningSquared[s];
-            tasks.push_back(t);
-        }
-    std::stable_sort(tasks.begin(), tasks.end(), [](const BatchTask& a, const BatchTask& b) { return a.nj > b.nj; });
-    e->ntasks = (int)tasks.size();
+    size_t ntasks = 0;
+    for (size_t s = 0; s < count; ++s) ntasks += (N[s] + kBatchTaskBodies - 1) / kBatchTaskBodies;
+    e->ntasks = (int)ntasks;
     e->blocks = d->cus * kBatchBlocksPerCU;
     size_t scratch = 0;
     e->plans.resize(count);
@@ -318,14 +333,21 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
     if (err == hipSuccess) err = dev_alloc(&e->pm, total);
     if (err == hipSuccess) err = dev_alloc(&e->vm, total);
     if (err == hipSuccess) err = dev_alloc(&e->force, total);
-    if (err == hipSuccess) err = dev_alloc(&e->tasks, tasks.size());
+    if (err == hipSuccess) err = dev_alloc(&e->tasks, ntasks);
     if (err == hipSuccess) err = dev_alloc(&e->counter, 1);
     if (err == hipSuccess) err = e->scratch.reserve(scratch);
     if (err == hipSuccess) err = hipMemcpy(e->pm, posMass, total * sizeof(float4), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(e->vm, velInvMass, total * sizeof(float4), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->force, 0, total * sizeof(float4));
-    if (err == hipSuccess)
+    if (err == hipSuccess) {
+        std::vector<BatchTask> tasks;
+        tasks.reserve(ntasks);
+        for (size_t s = 0; s < count; ++s)
+            append_tasks(tasks, e->pm + e->off[s], e->vm + e->off[s], e->force + e->off[s], N[s],
+                         softeningSquared[s]);
+        order_tasks(tasks);
         err = hipMemcpy(e->tasks, tasks.data(), tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice);
+    }
     if (err == hipSuccess) err = hipMemset(e->counter, 0, sizeof(unsigned));
     if (err == hipSuccess) err = e->timing.init();
     if (err != hipSuccess) {
@@ -392,6 +414,282 @@ void nbody_hip_ensemble_destroy(nbody_hip_ensemble* e)
     (void)hipStreamSynchronize(e->stream);
     e->release();
     delete e;
+}
+
+}  // extern "C"
+
+// ---- request server ----------------------------------------------------------
+
+namespace {
+
+struct ServeReq {
+    nbody_hip_request r{};
+    double admit = -1.0, finish = -1.0;
+    int done = 0;
+    int status = 0;
+    float4* buf = nullptr;  // [posMass | velInvMass | force], 3 N records
+    size_t bytes() const { return 3 * r.N * sizeof(float4); }
+};
+
+// Seeded initial conditions of a request (nbody_ics.h).
+int request_ics(const nbody_hip_request& r, std::vector<float>& pm, std::vector<float>& vm)
+{
+    pm.assign(4 * r.N, 0.f);
+    vm.assign(4 * r.N, 0.f);
+    if (r.ic == NBODY_IC_PLUMMER) return nbody_plummer(pm.data(), vm.data(), r.N, r.seed, 1.0f, 1.0f, 1);
+    const float radius = (float)(1.54 * std::pow((double)r.N / 16384.0, 1.0 / 3.0));
+    return nbody_uniform_ball(pm.data(), vm.data(), r.N, r.seed, radius, 1.0f, 1.0f);
+}
+
+}  // namespace
+
+struct nbody_hip_server {
+    size_t budget = 0;
+    int max_active = 1;
+    float dt = 0.f, damping = 1.f;
+    bool keep = false;
+    hipStream_t stream = nullptr;
+    unsigned* counter = nullptr;
+    BatchTask* tasks = nullptr;
+    size_t task_cap = 0;
+    int blocks = 0;
+    int peak_active = 0;
+    std::map<uint64_t, std::pair<std::vector<float>, std::vector<float>>> states;
+
+    void release()
+    {
+        if (tasks) (void)hipFree(tasks);
+        if (counter) (void)hipFree(counter);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+nbody_hip_server* nbody_hip_server_create(size_t budget_bytes, int max_active, float dt, float damping,
+                                          int keep_states)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (budget_bytes == 0 || max_active < 1) {
+        set_error("server needs a nonzero budget and at least one slot");
+        return nullptr;
+    }
+    DeviceState* d;
+    if (device_state(&d)) return nullptr;
+    auto* s = new nbody_hip_server();
+    s->budget = budget_bytes;
+    s->max_active = max_active;
+    s->dt = dt;
+    s->damping = damping;
+    s->keep = keep_states != 0;
+    s->blocks = d->cus * kBatchBlocksPerCU;
+    hipError_t e = hipStreamCreate(&s->stream);
+    if (e == hipSuccess) e = dev_alloc(&s->counter, 1);
+    if (e == hipSuccess) e = hipMemset(s->counter, 0, sizeof(unsigned));
+    if (e != hipSuccess) {
+        set_error("server create failed: %s", hipGetErrorString(e));
+        s->release();
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* reqs, size_t count,
+                               nbody_hip_result* results, double* wall_s, double* interactions)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!s || (count && (!reqs || !results))) {
+        set_error("null server or buffers");
+        return E_ARG;
+    }
+    for (size_t k = 0; k < count; ++k)
+        if (reqs[k].N < 1 || reqs[k].N > (size_t(1) << 24) || reqs[k].steps < 0 || !(reqs[k].arrival_s >= 0.0)) {
+            set_error("request %zu: N = %zu, steps = %d, arrival %g", k, reqs[k].N, reqs[k].steps, reqs[k].arrival_s);
+            return E_ARG;
+        }
+    std::vector<ServeReq> rq(count);
+    std::vector<size_t> order(count);
+    for (size_t k = 0; k < count; ++k) {
+        rq[k].r = reqs[k];
+        order[k] = k;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return rq[a].r.arrival_s < rq[b].r.arrival_s; });
+    s->states.clear();
+    s->peak_active = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    std::deque<ServeReq*> waiting;
+    std::vector<ServeReq*> active;
+    std::vector<BatchTask> host_tasks;
+    std::vector<float> pm, vm;
+    size_t next = 0, finished = 0, used = 0;
+    double work = 0.0;
+    bool dirty = true;
+    int rc = OK;
+    auto retire = [&](ServeReq* q, double t) {
+        q->finish = t;
+        if (q->buf) {
+            (void)hipFree(q->buf);
+            q->buf = nullptr;
+            used -= q->bytes();
+        }
+        ++finished;
+    };
+    while (finished < count && rc == OK) {
+        double t = now();
+        while (next < count && rq[order[next]].r.arrival_s <= t) waiting.push_back(&rq[order[next++]]);
+        // admit in arrival order while a slot is free and the budget holds
+        while (!waiting.empty() && (int)active.size() < s->max_active) {
+            ServeReq* q = waiting.front();
+            if (q->bytes() > s->budget) {
+                q->status = 1;
+                waiting.pop_front();
+                retire(q, now());
+                continue;
+            }
+            if (used + q->bytes() > s->budget) break;
+            if (request_ics(q->r, pm, vm)) {
+                q->status = 2;
+                waiting.pop_front();
+                retire(q, now());
+                continue;
+            }
+            hipError_t e = hipMalloc(reinterpret_cast<void**>(&q->buf), q->bytes());
+            if (e == hipSuccess) e = hipMemcpy(q->buf, pm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(q->buf + q->r.N, vm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                set_error("admitting request %llu failed: %s", (unsigned long long)q->r.id, hipGetErrorString(e));
+                if (q->buf) (void)hipFree(q->buf);
+                q->buf = nullptr;
+                rc = E_HIP;
+                break;
+            }
+            used += q->bytes();
+            q->admit = now();
+            waiting.pop_front();
+            if (q->r.steps == 0) {
+                retire(q, q->admit);
+                continue;
+            }
+            active.push_back(q);
+            dirty = true;
+        }
+        if (rc) break;
+        s->peak_active = std::max(s->peak_active, (int)active.size());
+        if (active.empty()) {
+            if (next < count) {
+                const double wait = rq[order[next]].r.arrival_s - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            }
+            continue;
+        }
+        if (dirty) {
+            host_tasks.clear();
+            for (ServeReq* q : active)
+                append_tasks(host_tasks, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->r.N, q->r.softeningSquared);
+            order_tasks(host_tasks);
+            if (host_tasks.size() > s->task_cap) {
+                if (s->tasks) (void)hipFree(s->tasks);
+                s->tasks = nullptr;
+                s->task_cap = 0;
+                if (hipError_t e = dev_alloc(&s->tasks, 2 * host_tasks.size()); e != hipSuccess) {
+                    set_error("task buffer: %s", hipGetErrorString(e));
+                    rc = E_HIP;
+                    break;
+                }
+                s->task_cap = 2 * host_tasks.size();
+            }
+            if (hipError_t e = hipMemcpy(s->tasks, host_tasks.data(), host_tasks.size() * sizeof(BatchTask),
+                                         hipMemcpyHostToDevice);
+                e != hipSuccess) {
+                set_error("task upload: %s", hipGetErrorString(e));
+                rc = E_HIP;
+                break;
+            }
+            dirty = false;
+        }
+        // one step of every active request, then wait for it
+        hipError_t e = launch_force_batch(s->tasks, (int)host_tasks.size(), s->counter, s->blocks, s->stream);
+        if (e == hipSuccess)
+            e = launch_integrate_batch(s->tasks, (int)host_tasks.size(), s->dt, s->damping, s->counter, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            set_error("server step: %s", hipGetErrorString(e));
+            rc = E_HIP;
+            break;
+        }
+        t = now();
+        std::vector<ServeReq*> still;
+        still.reserve(active.size());
+        for (ServeReq* q : active) {
+            work += (double)q->r.N * (double)q->r.N;
+            if (++q->done < q->r.steps) {
+                still.push_back(q);
+                continue;
+            }
+            if (s->keep) {
+                auto& st = s->states[q->r.id];
+                st.first.resize(4 * q->r.N);
+                st.second.resize(4 * q->r.N);
+                e = hipMemcpy(st.first.data(), q->buf, q->r.N * sizeof(float4), hipMemcpyDeviceToHost);
+                if (e == hipSuccess)
+                    e = hipMemcpy(st.second.data(), q->buf + q->r.N, q->r.N * sizeof(float4), hipMemcpyDeviceToHost);
+                if (e != hipSuccess) {
+                    set_error("reading request %llu: %s", (unsigned long long)q->r.id, hipGetErrorString(e));
+                    rc = E_HIP;
+                }
+            }
+            retire(q, t);
+            dirty = true;
+        }
+        active.swap(still);
+    }
+    for (ServeReq* q : active)
+        if (q->buf) (void)hipFree(q->buf);
+    for (size_t k = 0; k < count; ++k) {
+        const ServeReq& q = rq[k];
+        results[k].id = q.r.id;
+        results[k].N = q.r.N;
+        results[k].steps = q.r.steps;
+        results[k].status = q.finish < 0 ? 2 : q.status;
+        results[k].arrival_s = q.r.arrival_s;
+        results[k].admit_s = q.admit;
+        results[k].finish_s = q.finish;
+    }
+    if (wall_s) *wall_s = now();
+    if (interactions) *interactions = work;
+    return rc;
+}
+
+int nbody_hip_server_final_state(const nbody_hip_server* s, uint64_t id, float* posMass, float* velInvMass)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!s) {
+        set_error("null server");
+        return E_ARG;
+    }
+    auto it = s->states.find(id);
+    if (it == s->states.end()) {
+        set_error("no kept state for request %llu", (unsigned long long)id);
+        return E_ARG;
+    }
+    if (posMass) std::copy(it->second.first.begin(), it->second.first.end(), posMass);
+    if (velInvMass) std::copy(it->second.second.begin(), it->second.second.end(), velInvMass);
+    return OK;
+}
+
+int nbody_hip_server_peak_active(const nbody_hip_server* s) { return s ? s->peak_active : 0; }
+
+void nbody_hip_server_destroy(nbody_hip_server* s)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!s) return;
+    (void)hipStreamSynchronize(s->stream);
+    s->release();
+    delete s;
 }
 
 }  // extern "C"

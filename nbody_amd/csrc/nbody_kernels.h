@@ -76,30 +76,31 @@ namespace nbody {
 // ---- batches of independent systems (ensembles, served requests) ----------
 
 // One unit of the batched work queue: the force on bodies [i0, i0 + ni)
-// (ni <= 256 * ipl) of a system whose bodies are [j0, j0 + nj) in the
-// concatenated arrays. 32 bytes.
+// (ni <= 64) of a system of n bodies whose records are pm / vm / force.
+// Systems may live anywhere in device memory (ensemble slices, served
+// requests' own allocations). 40 bytes.
 struct BatchTask {
+    const float4* pm;
+    float4* vm;
+    float4* force;
     int i0;
     int ni;
-    int j0;
-    int nj;
+    int n;
     float eps2;
-    int pad[3];
 };
 
-// Largest ni a task may have for a given ipl.
-constexpr int batch_task_bodies(int ipl) { return 256 * ipl; }
+// Bodies per task: one wave of 64 lanes, one body per lane.
+constexpr int kBatchTaskBodies = 64;
 
-// Persistent force kernel: `blocks` workgroups take tasks from `counter`
-// (which must read 0 at launch) until all ntasks are done. Each body's force
-// depends only on its own system: TwoSum-folded 128-body chunks, the same
-// arithmetic whatever else is in the batch.
-hipError_t launch_force_batch(const float4* pm, float* force, const BatchTask* tasks, int ntasks, int ipl,
-                              unsigned* counter, int blocks, hipStream_t stream);
+// Persistent force kernel: `blocks` one-wave workgroups take tasks from
+// `counter` (which must read 0 at launch) until all ntasks are done. Each
+// body's force depends only on its own system: TwoSum-folded 128-body chunks,
+// the same arithmetic whatever else is in the batch.
+hipError_t launch_force_batch(const BatchTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream);
 
 // Integrate the bodies of every task (dt, damping as integrateGravitation)
 // and reset `counter` to 0 for the next force launch.
-hipError_t launch_integrate_batch(float4* pm, float4* vim, const float4* force, const BatchTask* tasks, int ntasks,
-                                  float dt, float damping, unsigned* counter, hipStream_t stream);
+hipError_t launch_integrate_batch(const BatchTask* tasks, int ntasks, float dt, float damping, unsigned* counter,
+                                  hipStream_t stream);
 
 }  // namespace nbody

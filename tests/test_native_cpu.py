@@ -124,3 +124,21 @@ def test_algorithm_table():
     with pytest.raises(KeyError):
         algorithms.find("GPU_Shared")
     assert nbody_amd.FLOPS_PER_INTERACTION == 20
+
+
+def test_poisson_trace_statistics():
+    from nbody_amd import serve
+    reqs = serve.poisson_trace(4000, 200.0, 1024, 131072, 10, 200, seed=9)
+    arr = np.array([r.arrival_s for r in reqs])
+    gaps = np.diff(arr)
+    assert arr[0] == 0.0 and np.all(gaps >= 0)
+    assert abs(gaps.mean() - 1 / 200.0) < 0.1 / 200.0          # mean inter-arrival 5 ms
+    assert abs(np.std(gaps) / gaps.mean() - 1.0) < 0.1         # exponential: cv = 1
+    n = np.array([r.N for r in reqs])
+    st = np.array([r.steps for r in reqs])
+    assert n.min() >= 1024 and n.max() <= 131072 and st.min() >= 10 and st.max() <= 200
+    assert abs(np.median(np.log(n)) - 0.5 * (np.log(1024) + np.log(131072))) < 0.2
+    ics_kind = np.array([r.ic for r in reqs])
+    assert 0.45 < ics_kind.mean() < 0.55
+    again = serve.poisson_trace(4000, 200.0, 1024, 131072, 10, 200, seed=9)
+    assert all(a.N == b.N and a.seed == b.seed and a.arrival_s == b.arrival_s for a, b in zip(reqs, again))
