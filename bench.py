@@ -48,9 +48,10 @@ def parse():
     ap.add_argument("--compensated", type=int, default=-1, choices=[-1, 0, 1])
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration for --pmc runs)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "served"],
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "served", "plummer"],
                     help="single: one N-body system (the headline line); ensemble: 512 independent systems; "
-                         "served: a Poisson stream of simulation requests")
+                         "served: a Poisson stream of simulation requests; plummer: N = 2^20 Plummer sphere, "
+                         "plain vs compensated sums")
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--max-active", type=int, default=64)
@@ -231,6 +232,68 @@ def bench_served(args):
     print(json.dumps(out))
 
 
+def bench_plummer(args):
+    """N = 1048576 Plummer sphere (a = 1, total mass 1, velInvMass.w = 1),
+    eps^2 = 1e-4, 5 steps (force + integrate, graph replay), for: the default
+    plan, the LDS kernel with TwoSum-compensated 128-body chunks, and the LDS
+    kernel with one plain FP32 sum over all N per body. Each plan's force on
+    the initial state is compared with the FP64 direct sum (GPU diagnostic)
+    over all N bodies."""
+    import numpy as np
+    import torch
+    from nbody_amd import _native, ics
+    L = _native.lib()
+    n = 1 << 20
+    eps2 = 1e-4
+    steps = args.steps if args.steps != 20 else 5
+    pm_h, vm_h = ics.plummer(n, seed=1)
+    dev = torch.device("cuda:0")
+    pm = torch.from_numpy(pm_h).to(dev)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    f64 = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    _native.check_ms(L.nbody_hip_force_fp64(P(f64), P(pm), eps2, n), "fp64")
+    ref = f64[:, :3]
+    ref_norm = ref.norm(dim=1)
+    plans = {
+        "default": _native.Options.make(),
+        "lds_compensated": _native.Options.make(variant=_native.LDS, compensated=1),
+        "lds_plain_single_chain": _native.Options.make(variant=_native.LDS, compensated=0, i_per_lane=2, jslice=n),
+    }
+    out_modes = {}
+    for name, opt in plans.items():
+        force = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+        _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, 0, n, ctypes.byref(opt), None), name)
+        torch.cuda.synchronize()
+        d = (force[:, :3].double() - ref).norm(dim=1) / ref_norm
+        h = L.nbody_hip_system_create(pm_h.ctypes.data, vm_h.ctypes.data, n, eps2, 0.016, 0.95, ctypes.byref(opt))
+        if not h:
+            raise _native.NativeError(_native.last_error())
+        try:
+            ms = ctypes.c_float()
+            _native.check(L.nbody_hip_system_run(h, 1, 1, ctypes.byref(ms)), name)
+            _native.check(L.nbody_hip_system_run(h, steps, 1, ctypes.byref(ms)), name)
+            plan = _native.Options()
+            L.nbody_hip_system_plan(h, ctypes.byref(plan))
+        finally:
+            L.nbody_hip_system_destroy(h)
+        out_modes[name] = {
+            "ms_per_step": round(ms.value / steps, 3),
+            "value": round(float(n) * n * steps / (ms.value * 1e-3) / 1e9, 1),
+            "max_rel": float(d.max()), "rms_rel": float(d.pow(2).mean().sqrt()),
+            "plan": {"variant": _native.VARIANT_NAMES[plan.variant], "i_per_lane": plan.i_per_lane,
+                     "jslice": plan.jslice, "compensated": bool(plan.compensated)},
+        }
+    best = out_modes["default"]
+    print(json.dumps({
+        "metric": METRIC, "value": best["value"], "unit": "10^9 interactions/s", "n_gpus": 1, "steps": steps,
+        "ms_per_step": best["ms_per_step"], "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic (seeded Plummer sphere)",
+        "config": {"workload": f"single system, N={n} Plummer sphere (a=1, M=1, invMass=1), eps^2={eps2}, "
+                               f"dt=0.016, damping=0.95, {steps} steps; errors vs FP64 direct sum over all N"},
+        "modes": out_modes,
+    }))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -259,6 +322,8 @@ def main():
         return bench_ensemble(args)
     if args.workload == "served":
         return bench_served(args)
+    if args.workload == "plummer":
+        return bench_plummer(args)
 
     L = _native.lib()
     n, K, W = args.n, args.steps, args.warmup

@@ -3,7 +3,8 @@
     python -m nbody_amd.build          # or nbody_amd.build.build()
 
 Outputs (git-ignored, shipped with the tree):
-    nbody_amd/lib/libnbody_hip.so   HIP kernels + the C ABI of include/nbody_hip.h
+    nbody_amd/lib/libnbody_hip.so   HIP kernels + the C ABI of include/*.h
+    nbody_amd/bin/nbody             the command-line driver (reference flags and output)
 Objects go to nbody_amd/_build/. Host-only C++ (initial conditions) is built
 with g++, strict IEEE. Every HIP translation unit is compiled for
 gfx950 with packed FP32 disabled (-Xclang -target-feature -Xclang -packed-fp32-ops):
@@ -20,6 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
+BINDIR = os.path.join(HERE, "bin")
 INCLUDE = os.path.join(ROOT, "include")
 
 ARCH = os.environ.get("NBODY_HIP_ARCH", "gfx950")
@@ -81,6 +83,13 @@ def build(verbose=False):
     tmp = lib + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *[j[0] for j in jobs]])
     os.replace(tmp, lib)
+    # the command-line driver (host code only), linked against the library
+    os.makedirs(BINDIR, exist_ok=True)
+    exe = os.path.join(BINDIR, "nbody")
+    _run([cc, "-O2", "-std=c++17", "-Wall", "-Wno-unused-command-line-argument", "-I", INCLUDE,
+          os.path.join(CSRC, "nbody_main.cpp"), "-o", exe + ".tmp", "-L", LIBDIR, "-lnbody_hip",
+          "-Wl,-rpath,$ORIGIN/../lib"])
+    os.replace(exe + ".tmp", exe)
     return lib
 
 

@@ -61,6 +61,12 @@ int device_state(DeviceState** out)
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+// Default symmetric plans aim for at most this many blocks (slot planes of
+// 12 N * blocks bytes), and NBODY_HIP_AUTO leaves the symmetric kernel for the
+// LDS one when the slots would exceed kSymScratchCap bytes.
+constexpr int kSymSlotsTarget = 512;
+constexpr size_t kSymScratchCap = size_t(8) << 30;
+
 // Default decomposition: enough blocks for ~48 waves per SIMD (the dispatcher
 // balances them across the 256 CUs), j slices of at least 1024 bodies in
 // multiples of 256.
@@ -72,10 +78,18 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
     const int comp_opt = opt ? opt->compensated : -1;
     p.comp = comp_opt < 0 ? (N > 65536) : comp_opt > 0;
     if (variant == VAR_SYMMETRIC) {
-        // blocks of B = 64 * ipl * waves (default 8 bodies per lane, one wave:
-        // B = 512); the block slots are always summed with TwoSum
+        // blocks of B = 64 * ipl * waves (default 8 bodies per lane and the
+        // fewest waves that keep at most kSymSlotsTarget blocks: B = 512 up to
+        // N = 256K, 2048 at N = 1M); the block slots are always summed with TwoSum
         p.ipl = (opt && opt->i_per_lane > 0) ? opt->i_per_lane : 8;
-        p.jslice = (opt && opt->jslice > 0) ? opt->jslice : (p.ipl == 2 ? 512 : 64 * p.ipl);
+        if (opt && opt->jslice > 0) {
+            p.jslice = opt->jslice;
+        } else if (p.ipl == 2) {
+            p.jslice = 512;
+        } else {
+            p.jslice = 64 * p.ipl;
+            while (p.jslice < 64 * p.ipl * 4 && (N + p.jslice - 1) / p.jslice > (size_t)kSymSlotsTarget) p.jslice *= 2;
+        }
         p.nslices = (int)((N + p.jslice - 1) / p.jslice);
         p.comp = true;
         return p;
@@ -107,7 +121,13 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
 // systems of at least two of its blocks (half the pairs, 16 VALU + 1 rsq per
 // pair); otherwise the LDS broadcast, whose VALU operands all sit in VGPRs
 // (gfx950 issues those faster than ops reading an SGPR).
-int auto_variant(size_t N, size_t n) { return (n == N && N >= 1024) ? VAR_SYMMETRIC : VAR_LDS; }
+int auto_variant(size_t N, size_t n)
+{
+    if (n != N || N < 1024) return VAR_LDS;
+    // the symmetric slots take 12 N^2 / B bytes; above kSymScratchCap use LDS
+    ForcePlan p = default_plan(VAR_SYMMETRIC, N, N, 1, nullptr);
+    return plan_scratch_bytes(p, (int)N) <= kSymScratchCap ? VAR_SYMMETRIC : VAR_LDS;
+}
 
 // Largest block count of a symmetric plan (the pair index must fit an int).
 constexpr int kMaxSymBlocks = 16384;
@@ -386,7 +406,8 @@ float nbody_hip_force_fp64(double* force64, const float* posMass, double softeni
     if (check_args(force64, posMass, N, 0, N)) return 0.0f;
     DeviceState* d;
     if (device_state(&d)) return 0.0f;
-    const int jslice = 2048;
+    // at most 64 slices: the FP64 partials take 24 bytes per body and slice
+    const int jslice = (int)std::max<size_t>(2048, (N + 63) / 64);
     const size_t nslices = (N + jslice - 1) / jslice;
     if (d->partial64.reserve(nslices * N * 3 * sizeof(double)) != hipSuccess) {
         set_error("cannot allocate FP64 scratch");
