@@ -6,7 +6,7 @@
  * The server keeps every admitted request's positions and velocities resident
  * in HBM and runs continuous batching: each iteration advances every active
  * request by one step with one persistent force launch over all of them (the
- * batched work queue of nbody_engine.h) and one integrate launch; between
+ * batched work queue of nbody_engine.h) and one reduce + integrate launch; between
  * iterations finished requests retire and waiting ones are admitted, in
  * arrival order, while a slot is free and their device memory fits the HBM
  * budget. A request's result depends only on the request, never on what else
@@ -50,11 +50,16 @@ typedef struct nbody_hip_result {
 
 typedef struct nbody_hip_server nbody_hip_server;
 
-/* budget_bytes: HBM the resident requests may hold (48 bytes per body);
+/* budget_bytes: HBM the resident requests may hold (48 bytes per body plus
+ * the batched force's block partials, as for nbody_hip_ensemble_create);
  * max_active: slots; dt, damping: the integrator's. keep_states != 0 keeps
  * each finished request's final posMass/velInvMass on the host. */
 nbody_hip_server *nbody_hip_server_create(size_t budget_bytes, int max_active, float dt, float damping,
                                           int keep_states);
+
+/* Device bytes a resident request of N bodies holds, the amount the budget
+ * counts: 48 N (posMass, velInvMass, force) plus its block partials. */
+size_t nbody_hip_request_bytes(size_t N);
 
 /* Replays a trace: request k enters the queue when arrival_s seconds have
  * passed since the call started; returns when every request has finished or

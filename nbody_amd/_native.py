@@ -103,6 +103,7 @@ SIGNATURES = {
                                         ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "nbody_hip_server_final_state": (_i, [_p, ctypes.c_uint64, _p, _p]),
     "nbody_hip_server_peak_active": (_i, [_p]),
+    "nbody_hip_request_bytes": (_sz, [_sz]),
     "nbody_hip_server_destroy": (None, [_p]),
     "nbody_random_unit_bodies": (_i, [_p, _p, _sz, _u32, _f, _f]),
     "nbody_uniform_ball": (_i, [_p, _p, _sz, _u32, _f, _f, _f]),

@@ -1,61 +1,115 @@
-// Batched force and integrate kernels for many independent systems in one
-// launch: a persistent grid of one-wave workgroups pulls (system, 64-body
-// block) tasks from a device work queue (an atomic counter); the caller
-// orders the tasks largest system first.
+// Batched force and integrate kernels for many independent systems: a
+// persistent grid of one-wave workgroups pulls ForceTasks (block pairs,
+// diagonal pieces, compensated rows) from a device work queue, then one
+// launch adds every body's slots and integrates it. Task lists are built on
+// the host, most expensive first.
 #include "nbody_device.h"
 #include "nbody_kernels.h"
+#include "nbody_sym.h"
+
+#include <algorithm>
 
 namespace nbody {
 
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kChunk = 128;
+constexpr int kChunk = 128;                  // j bodies between TwoSum folds of a ROW_COMP task
+constexpr int kPairA = kBatchBlock / kWave;  // bodies per lane of a PAIR task
 
-__global__ __launch_bounds__(kWave) void k_force_batch(const BatchTask* __restrict__ tasks, int ntasks,
+static_assert(kBatchBlock % kWave == 0 && kPairA == 8, "PAIR tasks are compiled for 8 bodies per lane");
+
+__global__ __launch_bounds__(kWave) void k_force_batch(const ForceTask* __restrict__ tasks, int ntasks,
                                                        unsigned* __restrict__ counter)
 {
+    __shared__ float jacc[3][kBatchBlock];
     for (;;) {
         int t = 0;
         if (threadIdx.x == 0) t = (int)atomicAdd(counter, 1u);
         // one wave per workgroup: lane 0's ticket is the wave's task, kept scalar
         t = __builtin_amdgcn_readfirstlane(t);
         if (t >= ntasks) return;
-        const BatchTask task = tasks[t];
-        const float4* __restrict__ pm = task.pm;
-        Lanes<1> me, hi, lo;
-        const float4 q = pm[task.i0 + min((int)threadIdx.x, task.ni - 1)];
+        const ForceTask task = tasks[t];
+        if (task.kind == TASK_PAIR) {
+            sym_pair<kPairA, 1, false>(task.pm, task.n, task.npad, task.eps2, task.part, task.a, task.b, jacc);
+            continue;
+        }
+        const int i = task.a + (int)threadIdx.x;
+        const float4 q = task.pm[min(i, task.n - 1)];
+        Lanes<1> me, hi;
         me.x[0] = q.x;
         me.y[0] = q.y;
         me.z[0] = q.z;
         zero<1>(hi);
-        zero<1>(lo);
-        for (int c = 0; c < task.n; c += kChunk) {
-            Lanes<1> part;
-            zero<1>(part);
-            sweep_const<1, false>(pm + c, min(kChunk, task.n - c), task.eps2, me, part);
-            fold<1>(hi, lo, part);
+        const float4* src = task.pm + task.b;
+        if (task.kind == TASK_ROW) {
+            sweep_const<1, false>(src, task.len, task.eps2, me, hi);
+        } else {
+            Lanes<1> lo;
+            zero<1>(lo);
+            for (int c = 0; c < task.len; c += kChunk) {
+                Lanes<1> part;
+                zero<1>(part);
+                sweep_const<1, false>(src + c, min(kChunk, task.len - c), task.eps2, me, part);
+                fold<1>(hi, lo, part);
+            }
+            hi.x[0] += lo.x[0];
+            hi.y[0] += lo.y[0];
+            hi.z[0] += lo.z[0];
         }
-        if ((int)threadIdx.x < task.ni) {
-            float* f = reinterpret_cast<float*>(task.force + task.i0 + threadIdx.x);
-            f[0] = hi.x[0] + lo.x[0];
-            f[1] = hi.y[0] + lo.y[0];
-            f[2] = hi.z[0] + lo.z[0];
+        if (i < task.n) {
+            const size_t plane = (size_t)task.nb * task.npad;
+            const size_t o = (size_t)task.slot * task.npad + i;
+            task.part[o] = hi.x[0];
+            task.part[plane + o] = hi.y[0];
+            task.part[2 * plane + o] = hi.z[0];
         }
     }
 }
 
-// One wave per task; same arithmetic as k_integrate (nbody_kernels.hip).
-__global__ __launch_bounds__(kWave) void k_integrate_batch(const BatchTask* __restrict__ tasks, float dt,
-                                                           float damping, unsigned* __restrict__ counter)
+// One wave per task. The slot sum is k_reduce_slots' (slot order, TwoSum,
+// rounded once) and the update is k_integrate's, both in nbody_kernels.hip.
+__global__ __launch_bounds__(kWave) void k_reduce_integrate_batch(const BodyTask* __restrict__ tasks, float dt,
+                                                                  float damping, unsigned* __restrict__ counter)
 {
+    constexpr int G = 8;
     if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
-    const BatchTask task = tasks[blockIdx.x];
+    const BodyTask task = tasks[blockIdx.x];
     if ((int)threadIdx.x >= task.ni) return;
     const int i = task.i0 + threadIdx.x;
+    const size_t plane = (size_t)task.nb * task.npad;
+    const float* src = task.part + i;
+    float s[3] = {0.f, 0.f, 0.f}, e[3] = {0.f, 0.f, 0.f};
+    int k = 0;
+    for (; k + G <= task.nb; k += G) {
+        float v[3][G];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int g = 0; g < G; ++g) v[c][g] = src[c * plane + (size_t)(k + g) * task.npad];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                float err;
+                s[c] = two_sum(s[c], v[c][g], err);
+                e[c] += err;
+            }
+    }
+    for (; k < task.nb; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float err;
+            s[c] = two_sum(s[c], src[c * plane + (size_t)k * task.npad], err);
+            e[c] += err;
+        }
+    float4 g = task.force[i];
+    g.x = s[0] + e[0];
+    g.y = s[1] + e[1];
+    g.z = s[2] + e[2];
+    task.force[i] = g;
     float4 p = task.pm[i];
     float4 v = task.vm[i];
-    const float4 g = task.force[i];
     v.x = __builtin_fmaf(v.w, g.x * dt, v.x);
     v.y = __builtin_fmaf(v.w, g.y * dt, v.y);
     v.z = __builtin_fmaf(v.w, g.z * dt, v.z);
@@ -65,14 +119,119 @@ __global__ __launch_bounds__(kWave) void k_integrate_batch(const BatchTask* __re
     p.x = __builtin_fmaf(v.x, dt, p.x);
     p.y = __builtin_fmaf(v.y, dt, p.y);
     p.z = __builtin_fmaf(v.z, dt, p.z);
-    // the batch's own integrate writes pm: it runs after the force launch has finished
-    const_cast<float4*>(task.pm)[i] = p;
+    task.pm[i] = p;
     task.vm[i] = v;
+}
+
+// Work of a task in VALU instructions per lane (16 + rsq per pair step,
+// 12 + rsq per one-sided interaction), for ordering only.
+double task_cost(const ForceTask& t)
+{
+    if (t.kind == TASK_PAIR) return (double)kBatchBlock * kBatchBlock / kWave * 17.0;
+    return (double)t.len * 13.0;
 }
 
 }  // namespace
 
-hipError_t launch_force_batch(const BatchTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream)
+void batch_layout(size_t n, int* nb, int* npad)
+{
+    if (n <= kBatchPairMaxN) {
+        *nb = (int)((n + kBatchBlock - 1) / kBatchBlock);
+        *npad = *nb * kBatchBlock;
+    } else {
+        *nb = 1;
+        *npad = (int)n;
+    }
+}
+
+size_t batch_part_floats(size_t n)
+{
+    int nb, npad;
+    batch_layout(n, &nb, &npad);
+    return (size_t)3 * nb * npad;
+}
+
+void append_force_tasks(std::vector<ForceTask>& out, const float4* pm, float* part, size_t n, float eps2)
+{
+    int nb, npad;
+    batch_layout(n, &nb, &npad);
+    ForceTask t{};
+    t.pm = pm;
+    t.part = part;
+    t.n = (int)n;
+    t.nb = nb;
+    t.npad = npad;
+    t.eps2 = eps2;
+    if (n > kBatchPairMaxN) {
+        t.kind = TASK_ROW_COMP;
+        t.b = 0;
+        t.len = (int)n;
+        t.slot = 0;
+        for (size_t i = 0; i < n; i += kWave) {
+            t.a = (int)i;
+            out.push_back(t);
+        }
+        return;
+    }
+    t.kind = TASK_PAIR;
+    for (int q = 1; q < nb; ++q)
+        for (int p = 0; p < q; ++p) {
+            t.a = p;
+            t.b = q;
+            out.push_back(t);
+        }
+    t.kind = TASK_ROW;
+    for (int p = 0; p < nb; ++p) {
+        const int j0 = p * kBatchBlock;
+        t.b = j0;
+        t.len = (int)std::min<size_t>(kBatchBlock, n - j0);
+        t.slot = p;
+        for (int i = j0; i < j0 + t.len; i += kWave) {
+            t.a = i;
+            out.push_back(t);
+        }
+    }
+}
+
+void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float4* force, const float* part,
+                       size_t n)
+{
+    int nb, npad;
+    batch_layout(n, &nb, &npad);
+    for (size_t i = 0; i < n; i += kWave) {
+        BodyTask t{};
+        t.pm = pm;
+        t.vm = vm;
+        t.force = force;
+        t.part = part;
+        t.i0 = (int)i;
+        t.ni = (int)std::min<size_t>(kWave, n - i);
+        t.nb = nb;
+        t.npad = npad;
+        out.push_back(t);
+    }
+}
+
+void order_force_tasks(std::vector<ForceTask>& tasks)
+{
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [](const ForceTask& a, const ForceTask& b) { return task_cost(a) > task_cost(b); });
+}
+
+int batch_force_blocks_per_cu()
+{
+    static int cached = 0;
+    if (cached == 0) {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_force_batch, kWave, 0) != hipSuccess ||
+            blocks < 1)
+            blocks = 16;
+        cached = blocks;
+    }
+    return cached;
+}
+
+hipError_t launch_force_batch(const ForceTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream)
 {
     if (ntasks <= 0) return hipSuccess;
     blocks = blocks < ntasks ? blocks : ntasks;
@@ -80,11 +239,11 @@ hipError_t launch_force_batch(const BatchTask* tasks, int ntasks, unsigned* coun
     return hipGetLastError();
 }
 
-hipError_t launch_integrate_batch(const BatchTask* tasks, int ntasks, float dt, float damping, unsigned* counter,
-                                  hipStream_t stream)
+hipError_t launch_reduce_integrate_batch(const BodyTask* tasks, int ntasks, float dt, float damping,
+                                         unsigned* counter, hipStream_t stream)
 {
     if (ntasks <= 0) return hipSuccess;
-    k_integrate_batch<<<ntasks, kWave, 0, stream>>>(tasks, dt, damping, counter);
+    k_reduce_integrate_batch<<<ntasks, kWave, 0, stream>>>(tasks, dt, damping, counter);
     return hipGetLastError();
 }
 

@@ -24,37 +24,32 @@ using namespace nbody::detail;
 
 namespace {
 
-// One-wave workgroups of the persistent batch kernel per CU: its ~50 VGPRs
-// allow 8 waves per SIMD, 32 per CU.
-constexpr int kBatchBlocksPerCU = 32;
-
-// The (system, 64-body block) tasks of one system.
-void append_tasks(std::vector<BatchTask>& out, float4* pm, float4* vm, float4* force, size_t n, float eps2)
-{
-    for (size_t b = 0; b < n; b += kBatchTaskBodies) {
-        BatchTask t{};
-        t.pm = pm;
-        t.vm = vm;
-        t.force = force;
-        t.i0 = (int)b;
-        t.ni = (int)std::min<size_t>(kBatchTaskBodies, n - b);
-        t.n = (int)n;
-        t.eps2 = eps2;
-        out.push_back(t);
-    }
-}
-
-// Largest systems first, so the tail of the persistent launch holds the
-// cheapest tasks.
-void order_tasks(std::vector<BatchTask>& tasks)
-{
-    std::stable_sort(tasks.begin(), tasks.end(), [](const BatchTask& a, const BatchTask& b) { return a.n > b.n; });
-}
-
 template <class T>
 hipError_t dev_alloc(T** p, size_t count)
 {
     return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
+}
+
+// Copy a task list to a device buffer that grows (to twice the need) when short.
+template <class T>
+int upload(T** dev, size_t* cap, const std::vector<T>& host)
+{
+    if (host.size() > *cap) {
+        if (*dev) (void)hipFree(*dev);
+        *dev = nullptr;
+        *cap = 0;
+        if (hipError_t e = dev_alloc(dev, 2 * host.size()); e != hipSuccess) {
+            set_error("task buffer: %s", hipGetErrorString(e));
+            return E_HIP;
+        }
+        *cap = 2 * host.size();
+    }
+    if (host.empty()) return OK;
+    if (hipError_t e = hipMemcpy(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice); e != hipSuccess) {
+        set_error("task upload: %s", hipGetErrorString(e));
+        return E_HIP;
+    }
+    return OK;
 }
 
 struct Timing {
@@ -130,9 +125,11 @@ struct nbody_hip_ensemble {
     float4* pm = nullptr;
     float4* vm = nullptr;
     float4* force = nullptr;
-    BatchTask* tasks = nullptr;
+    float* part = nullptr;         // every system's slot planes, back to back
+    ForceTask* ftasks = nullptr;
+    BodyTask* btasks = nullptr;
     unsigned* counter = nullptr;
-    int ntasks = 0;
+    int nft = 0, nbt = 0;
     int blocks = 0;
     std::vector<ForcePlan> plans;  // per-system mode
     Buffer scratch;
@@ -144,7 +141,8 @@ struct nbody_hip_ensemble {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* p : {(void*)pm, (void*)vm, (void*)force, (void*)tasks, (void*)counter, scratch.ptr})
+        for (void* p : {(void*)pm, (void*)vm, (void*)force, (void*)part, (void*)ftasks, (void*)btasks, (void*)counter,
+                        scratch.ptr})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
         timing.destroy();
@@ -152,8 +150,8 @@ struct nbody_hip_ensemble {
 
     int queue_batched()
     {
-        HIP_TRY(launch_force_batch(tasks, ntasks, counter, blocks, stream));
-        HIP_TRY(launch_integrate_batch(tasks, ntasks, dt, damping, counter, stream));
+        HIP_TRY(launch_force_batch(ftasks, nft, counter, blocks, stream));
+        HIP_TRY(launch_reduce_integrate_batch(btasks, nbt, dt, damping, counter, stream));
         return OK;
     }
 
@@ -316,10 +314,13 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
         delete e;
         return nullptr;
     }
-    size_t ntasks = 0;
-    for (size_t s = 0; s < count; ++s) ntasks += (N[s] + kBatchTaskBodies - 1) / kBatchTaskBodies;
-    e->ntasks = (int)ntasks;
-    e->blocks = d->cus * kBatchBlocksPerCU;
+    std::vector<size_t> part_off(count);
+    size_t part_total = 0;
+    for (size_t s = 0; s < count; ++s) {
+        part_off[s] = part_total;
+        part_total += batch_part_floats(N[s]);
+    }
+    e->blocks = d->cus * batch_force_blocks_per_cu();
     size_t scratch = 0;
     e->plans.resize(count);
     for (size_t s = 0; s < count; ++s) {
@@ -333,20 +334,29 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
     if (err == hipSuccess) err = dev_alloc(&e->pm, total);
     if (err == hipSuccess) err = dev_alloc(&e->vm, total);
     if (err == hipSuccess) err = dev_alloc(&e->force, total);
-    if (err == hipSuccess) err = dev_alloc(&e->tasks, ntasks);
+    if (err == hipSuccess) err = dev_alloc(&e->part, part_total);
     if (err == hipSuccess) err = dev_alloc(&e->counter, 1);
     if (err == hipSuccess) err = e->scratch.reserve(scratch);
     if (err == hipSuccess) err = hipMemcpy(e->pm, posMass, total * sizeof(float4), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(e->vm, velInvMass, total * sizeof(float4), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->force, 0, total * sizeof(float4));
     if (err == hipSuccess) {
-        std::vector<BatchTask> tasks;
-        tasks.reserve(ntasks);
-        for (size_t s = 0; s < count; ++s)
-            append_tasks(tasks, e->pm + e->off[s], e->vm + e->off[s], e->force + e->off[s], N[s],
-                         softeningSquared[s]);
-        order_tasks(tasks);
-        err = hipMemcpy(e->tasks, tasks.data(), tasks.size() * sizeof(BatchTask), hipMemcpyHostToDevice);
+        std::vector<ForceTask> ft;
+        std::vector<BodyTask> bt;
+        for (size_t s = 0; s < count; ++s) {
+            append_force_tasks(ft, e->pm + e->off[s], e->part + part_off[s], N[s], softeningSquared[s]);
+            append_body_tasks(bt, e->pm + e->off[s], e->vm + e->off[s], e->force + e->off[s], e->part + part_off[s],
+                              N[s]);
+        }
+        order_force_tasks(ft);
+        e->nft = (int)ft.size();
+        e->nbt = (int)bt.size();
+        err = dev_alloc(&e->ftasks, ft.size());
+        if (err == hipSuccess) err = dev_alloc(&e->btasks, bt.size());
+        if (err == hipSuccess)
+            err = hipMemcpy(e->ftasks, ft.data(), ft.size() * sizeof(ForceTask), hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = hipMemcpy(e->btasks, bt.data(), bt.size() * sizeof(BodyTask), hipMemcpyHostToDevice);
     }
     if (err == hipSuccess) err = hipMemset(e->counter, 0, sizeof(unsigned));
     if (err == hipSuccess) err = e->timing.init();
@@ -427,8 +437,9 @@ struct ServeReq {
     double admit = -1.0, finish = -1.0;
     int done = 0;
     int status = 0;
-    float4* buf = nullptr;  // [posMass | velInvMass | force], 3 N records
-    size_t bytes() const { return 3 * r.N * sizeof(float4); }
+    float4* buf = nullptr;  // [posMass | velInvMass | force], 3 N records, then the slot planes
+    size_t bytes() const { return 3 * r.N * sizeof(float4) + batch_part_floats(r.N) * sizeof(float); }
+    float* part() const { return reinterpret_cast<float*>(buf + 3 * r.N); }
 };
 
 // Seeded initial conditions of a request (nbody_ics.h).
@@ -450,15 +461,17 @@ struct nbody_hip_server {
     bool keep = false;
     hipStream_t stream = nullptr;
     unsigned* counter = nullptr;
-    BatchTask* tasks = nullptr;
-    size_t task_cap = 0;
+    ForceTask* ftasks = nullptr;
+    BodyTask* btasks = nullptr;
+    size_t ft_cap = 0, bt_cap = 0;
     int blocks = 0;
     int peak_active = 0;
     std::map<uint64_t, std::pair<std::vector<float>, std::vector<float>>> states;
 
     void release()
     {
-        if (tasks) (void)hipFree(tasks);
+        if (ftasks) (void)hipFree(ftasks);
+        if (btasks) (void)hipFree(btasks);
         if (counter) (void)hipFree(counter);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -482,7 +495,7 @@ nbody_hip_server* nbody_hip_server_create(size_t budget_bytes, int max_active, f
     s->dt = dt;
     s->damping = damping;
     s->keep = keep_states != 0;
-    s->blocks = d->cus * kBatchBlocksPerCU;
+    s->blocks = d->cus * batch_force_blocks_per_cu();
     hipError_t e = hipStreamCreate(&s->stream);
     if (e == hipSuccess) e = dev_alloc(&s->counter, 1);
     if (e == hipSuccess) e = hipMemset(s->counter, 0, sizeof(unsigned));
@@ -522,7 +535,8 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
     std::deque<ServeReq*> waiting;
     std::vector<ServeReq*> active;
-    std::vector<BatchTask> host_tasks;
+    std::vector<ForceTask> host_ft;
+    std::vector<BodyTask> host_bt;
     std::vector<float> pm, vm;
     size_t next = 0, finished = 0, used = 0;
     double work = 0.0;
@@ -587,34 +601,28 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             continue;
         }
         if (dirty) {
-            host_tasks.clear();
-            for (ServeReq* q : active)
-                append_tasks(host_tasks, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->r.N, q->r.softeningSquared);
-            order_tasks(host_tasks);
-            if (host_tasks.size() > s->task_cap) {
-                if (s->tasks) (void)hipFree(s->tasks);
-                s->tasks = nullptr;
-                s->task_cap = 0;
-                if (hipError_t e = dev_alloc(&s->tasks, 2 * host_tasks.size()); e != hipSuccess) {
-                    set_error("task buffer: %s", hipGetErrorString(e));
-                    rc = E_HIP;
-                    break;
-                }
-                s->task_cap = 2 * host_tasks.size();
+            host_ft.clear();
+            host_bt.clear();
+            for (ServeReq* q : active) {
+                append_force_tasks(host_ft, q->buf, q->part(), q->r.N, q->r.softeningSquared);
+                append_body_tasks(host_bt, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->part(), q->r.N);
             }
-            if (hipError_t e = hipMemcpy(s->tasks, host_tasks.data(), host_tasks.size() * sizeof(BatchTask),
-                                         hipMemcpyHostToDevice);
-                e != hipSuccess) {
-                set_error("task upload: %s", hipGetErrorString(e));
-                rc = E_HIP;
+            order_force_tasks(host_ft);
+            if (int r = upload(&s->ftasks, &s->ft_cap, host_ft); r) {
+                rc = r;
+                break;
+            }
+            if (int r = upload(&s->btasks, &s->bt_cap, host_bt); r) {
+                rc = r;
                 break;
             }
             dirty = false;
         }
         // one step of every active request, then wait for it
-        hipError_t e = launch_force_batch(s->tasks, (int)host_tasks.size(), s->counter, s->blocks, s->stream);
+        hipError_t e = launch_force_batch(s->ftasks, (int)host_ft.size(), s->counter, s->blocks, s->stream);
         if (e == hipSuccess)
-            e = launch_integrate_batch(s->tasks, (int)host_tasks.size(), s->dt, s->damping, s->counter, s->stream);
+            e = launch_reduce_integrate_batch(s->btasks, (int)host_bt.size(), s->dt, s->damping, s->counter,
+                                              s->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) {
             set_error("server step: %s", hipGetErrorString(e));
@@ -682,6 +690,13 @@ int nbody_hip_server_final_state(const nbody_hip_server* s, uint64_t id, float* 
 }
 
 int nbody_hip_server_peak_active(const nbody_hip_server* s) { return s ? s->peak_active : 0; }
+
+size_t nbody_hip_request_bytes(size_t N)
+{
+    ServeReq q;
+    q.r.N = N;
+    return q.bytes();
+}
 
 void nbody_hip_server_destroy(nbody_hip_server* s)
 {

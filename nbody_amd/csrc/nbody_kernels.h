@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <vector>
 
 namespace nbody {
 
@@ -74,33 +75,74 @@ hipError_t launch_energy_fp64(double* acc, const float4* pm, const float4* vim, 
 namespace nbody {
 
 // ---- batches of independent systems (ensembles, served requests) ----------
+//
+// One persistent launch computes the forces of many systems: one-wave
+// workgroups take ForceTasks from a device work queue (an atomic counter).
+// A system of n <= kBatchPairMaxN bodies is cut into blocks of kBatchBlock
+// and runs as the symmetric plan does (nbody_sym.h): one PAIR task per block
+// pair P < Q and kBatchBlock/64 one-sided ROW pieces per diagonal block, each
+// writing its own slot of the system's partial planes part[3][nb][npad]. A
+// larger system (its slot planes would grow as n^2/kBatchBlock) runs as
+// ROW_COMP tasks: 64 bodies against all n with TwoSum-folded chunks, into a
+// single slot. A second launch adds each body's slots in slot order (TwoSum)
+// and integrates it. A body's result therefore depends only on its own
+// system, never on what else is in the batch or on scheduling.
 
-// One unit of the batched work queue: the force on bodies [i0, i0 + ni)
-// (ni <= 64) of a system of n bodies whose records are pm / vm / force.
-// Systems may live anywhere in device memory (ensemble slices, served
-// requests' own allocations). 40 bytes.
-struct BatchTask {
-    const float4* pm;
-    float4* vm;
-    float4* force;
-    int i0;
-    int ni;
-    int n;
+constexpr int kBatchBlock = 512;                     // 8 bodies per lane, one wave
+constexpr size_t kBatchPairMaxN = size_t(1) << 18;   // largest system run as block pairs
+
+enum BatchTaskKind : int { TASK_PAIR = 0, TASK_ROW = 1, TASK_ROW_COMP = 2 };
+
+struct ForceTask {
+    const float4* pm;  // the system's bodies
+    float* part;       // the system's slot planes [3][nb][npad]
+    int n;             // bodies in the system
+    int nb;            // slots per body
+    int npad;          // length of one slot row
     float eps2;
+    int kind;          // BatchTaskKind
+    int a;             // PAIR: block P; ROW*: first i body (64 of them)
+    int b;             // PAIR: block Q; ROW*: first j body
+    int len;           // ROW*: number of j bodies
+    int slot;          // ROW*: slot written
+    int reserved;
 };
 
-// Bodies per task: one wave of 64 lanes, one body per lane.
-constexpr int kBatchTaskBodies = 64;
+// Reduction + integration of up to 64 consecutive bodies of one system.
+struct BodyTask {
+    float4* pm;
+    float4* vm;
+    float4* force;
+    const float* part;
+    int i0;
+    int ni;
+    int nb;
+    int npad;
+};
 
-// Persistent force kernel: `blocks` one-wave workgroups take tasks from
-// `counter` (which must read 0 at launch) until all ntasks are done. Each
-// body's force depends only on its own system: TwoSum-folded 128-body chunks,
-// the same arithmetic whatever else is in the batch.
-hipError_t launch_force_batch(const BatchTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream);
+// Slots per body and slot row length of a batched system of n bodies.
+void batch_layout(size_t n, int* nb, int* npad);
+// Floats of slot planes a batched system of n bodies needs (3 * nb * npad).
+size_t batch_part_floats(size_t n);
+// Append one system's force tasks / body tasks.
+void append_force_tasks(std::vector<ForceTask>& out, const float4* pm, float* part, size_t n, float eps2);
+void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float4* force, const float* part,
+                       size_t n);
+// Most expensive tasks first (stable), so the end of the launch holds the
+// cheapest ones.
+void order_force_tasks(std::vector<ForceTask>& tasks);
+// Resident one-wave force workgroups per CU (the persistent grid is this
+// times the CU count).
+int batch_force_blocks_per_cu();
 
-// Integrate the bodies of every task (dt, damping as integrateGravitation)
-// and reset `counter` to 0 for the next force launch.
-hipError_t launch_integrate_batch(const BatchTask* tasks, int ntasks, float dt, float damping, unsigned* counter,
-                                  hipStream_t stream);
+// Persistent force launch: `blocks` workgroups take tasks from `counter`
+// (which must read 0 at launch) until all ntasks are done.
+hipError_t launch_force_batch(const ForceTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream);
+
+// Sum every body's slots, write force (x, y, z; w untouched), integrate
+// (dt, damping as integrateGravitation), and reset `counter` to 0 for the
+// next force launch.
+hipError_t launch_reduce_integrate_batch(const BodyTask* tasks, int ntasks, float dt, float damping,
+                                         unsigned* counter, hipStream_t stream);
 
 }  // namespace nbody

@@ -14,6 +14,7 @@
 // x, y, z only; w is left alone (the contract of src/nbody.cu:351-371).
 #include "nbody_device.h"
 #include "nbody_kernels.h"
+#include "nbody_sym.h"
 
 namespace nbody {
 
@@ -197,22 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_force_dpp(ForceParams p)
 
 // ---- symmetric: each pair once, reaction kept on chip ----------------------
 //
-// Bodies are grouped in blocks of B = 64*A*W. Workgroup t computes block pair
-// (P, Q), P <= Q, t = Q(Q+1)/2 + P. Its W waves hold A bodies of block P per
-// lane. For P < Q each wave streams block Q as 64-body tiles: lane l loads one
-// j body, and after every step the j body's position, mass and reaction
-// accumulator move one lane along the wave (DPP wave_ror:1), so in 64 steps
-// each lane meets all 64 and the accumulators come back home. Per pair:
-// 16 VALU + 1 v_rsq_f32 for both directions (the reaction on j carries m_i,
-// where the reference's symmetric kernels use m_j: src/nbody_GPU_AOS_tiled.cu:133-141).
-// Waves visit the tiles staggered (wave w takes tile (phase + w*A) mod T), so
-// within a phase no two waves own a tile and the per-tile reaction sums enter
-// LDS in a fixed order, with one barrier per phase: no atomics anywhere.
-// Diagonal pairs (P == Q) run the plain i-against-all-j sweep of the block.
-// Every body then has exactly one partial per block slot (slot Q for the i
-// side of (P, Q), slot P for its j side, slot P for (P, P)); k_reduce_slots
-// adds the slots in order with TwoSum.
-
+// The symmetric plan (see nbody_sym.h): workgroup t < nb(nb-1)/2 computes
+// block pair (P, Q), P < Q; the remaining nb*A workgroups are the diagonal
+// pieces, last in the grid so that they fill the end of the launch.
 struct SymParams {
     const float4* pm;
     int n;
@@ -222,106 +210,20 @@ struct SymParams {
     float* part; // three planes [3][nb][npad]
 };
 
-// Every lane reads the lane below it, lane 0 reads lane 63 (DPP wave_ror:1).
-// All source lanes are valid, so no "old" value has to be kept.
-__device__ __forceinline__ float wave_ror1(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xf, 0xf, true));
-}
-
-__device__ __forceinline__ void task_pair(int t, int& P, int& Q)
-{
-    int q = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-    while (q > 0 && q * (q + 1) / 2 > t) --q;
-    while ((q + 1) * (q + 2) / 2 <= t) ++q;
-    Q = q;
-    P = t - q * (q + 1) / 2;
-}
-
 template <int A, int W, bool NEWTON>
 __global__ __launch_bounds__(64 * W) void k_force_sym(SymParams p)
 {
     constexpr int B = 64 * A * W;
-    constexpr int T = A * W;
     __shared__ float jacc[3][B];
-    int P, Q;
-    task_pair(blockIdx.x, P, Q);
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const size_t plane = (size_t)p.nb * p.npad;
-
-    Lanes<A> me, acc;
-    float nm[A];
-#pragma unroll
-    for (int u = 0; u < A; ++u) {
-        const int idx = P * B + u * 64 * W + threadIdx.x;
-        const float4 q = idx < p.n ? p.pm[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
-        me.x[u] = q.x;
-        me.y[u] = q.y;
-        me.z[u] = q.z;
-        nm[u] = -q.w;
-    }
-    zero<A>(acc);
-
-    if (P == Q) {
-        const int j0 = P * B;
-        sweep_sgpr<A, NEWTON>(p.pm + j0, min(p.n - j0, B), p.eps2, me, acc);
+    const int npairs = p.nb * (p.nb - 1) / 2;
+    const int t = blockIdx.x;
+    if (t < npairs) {
+        int P, Q;
+        upper_pair(t, P, Q);
+        sym_pair<A, W, NEWTON>(p.pm, p.n, p.npad, p.eps2, p.part, P, Q, jacc);
     } else {
-        for (int k = threadIdx.x; k < B; k += 64 * W) jacc[0][k] = jacc[1][k] = jacc[2][k] = 0.f;
-        __syncthreads();
-        for (int ph = 0; ph < T; ++ph) {
-            const int tile = (ph + wave * A) % T;
-            const int jidx = Q * B + tile * 64 + lane;
-            const float4 b = jidx < p.n ? p.pm[jidx] : make_float4(0.f, 0.f, 0.f, 0.f);
-            float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
-            float rx = 0.f, ry = 0.f, rz = 0.f;
-            for (int k = 0; k < 64; ++k) {
-#pragma unroll
-                for (int u = 0; u < A; ++u) {
-                    const float dx = xj - me.x[u];
-                    const float dy = yj - me.y[u];
-                    const float dz = zj - me.z[u];
-                    float d2 = __builtin_fmaf(dx, dx, p.eps2);
-                    d2 = __builtin_fmaf(dy, dy, d2);
-                    d2 = __builtin_fmaf(dz, dz, d2);
-                    const float inv = rsqrt_f32<NEWTON>(d2);
-                    const float inv3 = (inv * inv) * inv;
-                    const float si = mj * inv3;
-                    const float sj = nm[u] * inv3;
-                    acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
-                    acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
-                    acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
-                    rx = __builtin_fmaf(dx, sj, rx);
-                    ry = __builtin_fmaf(dy, sj, ry);
-                    rz = __builtin_fmaf(dz, sj, rz);
-                }
-                xj = wave_ror1(xj);
-                yj = wave_ror1(yj);
-                zj = wave_ror1(zj);
-                mj = wave_ror1(mj);
-                rx = wave_ror1(rx);
-                ry = wave_ror1(ry);
-                rz = wave_ror1(rz);
-            }
-            const int slot = tile * 64 + lane;
-            jacc[0][slot] += rx;
-            jacc[1][slot] += ry;
-            jacc[2][slot] += rz;
-            __syncthreads();
-        }
-        for (int k = threadIdx.x; k < B; k += 64 * W) {
-            const size_t o = (size_t)P * p.npad + (size_t)Q * B + k;
-            p.part[o] = jacc[0][k];
-            p.part[plane + o] = jacc[1][k];
-            p.part[2 * plane + o] = jacc[2][k];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < A; ++u) {
-        const size_t o = (size_t)Q * p.npad + (size_t)P * B + u * 64 * W + threadIdx.x;
-        p.part[o] = acc.x[u];
-        p.part[plane + o] = acc.y[u];
-        p.part[2 * plane + o] = acc.z[u];
+        const int d = t - npairs;
+        sym_diag_piece<B, NEWTON, false>(p.pm, p.n, p.npad, p.eps2, p.part, d / A, d % A, 64 * W);
     }
 }
 
@@ -548,7 +450,7 @@ hipError_t go_family(const ForcePlan& plan, dim3 g, hipStream_t s, const ForcePa
 template <int A, int W>
 hipError_t go_sym(const ForcePlan& plan, const SymParams& sp, hipStream_t s)
 {
-    const int tasks = sp.nb * (sp.nb + 1) / 2;
+    const int tasks = sp.nb * (sp.nb - 1) / 2 + sp.nb * A;
     if (plan.newton) k_force_sym<A, W, true><<<tasks, 64 * W, 0, s>>>(sp);
     else k_force_sym<A, W, false><<<tasks, 64 * W, 0, s>>>(sp);
     return hipGetLastError();

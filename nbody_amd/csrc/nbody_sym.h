@@ -1,0 +1,157 @@
+// The symmetric decomposition's device code, shared by the single-system
+// kernel (k_force_sym, nbody_kernels.hip) and the batched work-queue kernel
+// (k_force_batch, nbody_batch.hip).
+//
+// Bodies are grouped in blocks of B = 64*A*W. A block pair (P, Q), P < Q, is
+// one workgroup task: its W waves hold A bodies of block P per lane and stream
+// block Q as 64-body tiles. Lane l loads one j body; after every step the j
+// body's position, mass and reaction accumulator move one lane along the wave
+// (DPP wave_ror:1), so in 64 steps each lane meets all 64 and the
+// accumulators come back home. Per pair: 16 VALU + 1 v_rsq_f32 for both
+// directions (the reaction on j carries m_i, where the reference's symmetric
+// kernels use m_j: src/nbody_GPU_AOS_tiled.cu:133-141). Waves visit the tiles
+// staggered (wave w takes tile (phase + w*A) mod T), so within a phase no two
+// waves own a tile and the per-tile reaction sums enter LDS in a fixed order,
+// with one barrier per phase: no atomics anywhere.
+//
+// The diagonal of block P (its bodies against each other) is one-sided and
+// split into A pieces of 64*W bodies each, scheduled after every pair so that
+// these short tasks fill the end of the launch.
+//
+// Every body then has exactly one partial per block slot: slot Q for the i
+// side of (P, Q), slot P for its j side, slot P for the diagonal. The partials
+// live in three float planes part[3][nb][npad] (npad = nb*B) and are added in
+// slot order with TwoSum by the reduction kernels.
+#pragma once
+
+#include "nbody_device.h"
+
+namespace nbody {
+
+// Every lane reads the lane below it, lane 0 reads lane 63 (DPP wave_ror:1).
+// All source lanes are valid, so no "old" value has to be kept.
+__device__ __forceinline__ float wave_ror1(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xf, 0xf, true));
+}
+
+// Task t < nb(nb-1)/2 of the strictly upper block triangle: t = Q(Q-1)/2 + P, P < Q.
+__device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
+{
+    int q = (int)((sqrtf(8.0f * (float)t + 1.0f) + 1.0f) * 0.5f);
+    while (q > 1 && q * (q - 1) / 2 > t) --q;
+    while ((q + 1) * q / 2 <= t) ++q;
+    Q = q;
+    P = t - q * (q - 1) / 2;
+}
+
+// Block pair (P, Q), P < Q. `jacc` is the workgroup's LDS reaction buffer.
+// P is never the last (possibly ragged) block, so every i lane holds a real
+// body; j bodies past n enter with zero mass.
+template <int A, int W, bool NEWTON>
+__device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, int npad, float eps2,
+                                         float* __restrict__ part, int P, int Q, float (*jacc)[64 * A * W])
+{
+    constexpr int B = 64 * A * W;
+    constexpr int T = A * W;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const size_t plane = (size_t)(npad / B) * npad;
+
+    Lanes<A> me, acc;
+    float nm[A];
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+        const float4 q = pm[P * B + u * 64 * W + threadIdx.x];
+        me.x[u] = q.x;
+        me.y[u] = q.y;
+        me.z[u] = q.z;
+        nm[u] = -q.w;
+    }
+    zero<A>(acc);
+
+    for (int k = threadIdx.x; k < B; k += 64 * W) jacc[0][k] = jacc[1][k] = jacc[2][k] = 0.f;
+    __syncthreads();
+    for (int ph = 0; ph < T; ++ph) {
+        const int tile = (ph + wave * A) % T;
+        const int jidx = Q * B + tile * 64 + lane;
+        const float4 b = jidx < n ? pm[jidx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
+        float rx = 0.f, ry = 0.f, rz = 0.f;
+        for (int k = 0; k < 64; ++k) {
+#pragma unroll
+            for (int u = 0; u < A; ++u) {
+                const float dx = xj - me.x[u];
+                const float dy = yj - me.y[u];
+                const float dz = zj - me.z[u];
+                float d2 = __builtin_fmaf(dx, dx, eps2);
+                d2 = __builtin_fmaf(dy, dy, d2);
+                d2 = __builtin_fmaf(dz, dz, d2);
+                const float inv = rsqrt_f32<NEWTON>(d2);
+                const float inv3 = (inv * inv) * inv;
+                const float si = mj * inv3;
+                const float sj = nm[u] * inv3;
+                acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
+                acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
+                acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
+                rx = __builtin_fmaf(dx, sj, rx);
+                ry = __builtin_fmaf(dy, sj, ry);
+                rz = __builtin_fmaf(dz, sj, rz);
+            }
+            xj = wave_ror1(xj);
+            yj = wave_ror1(yj);
+            zj = wave_ror1(zj);
+            mj = wave_ror1(mj);
+            rx = wave_ror1(rx);
+            ry = wave_ror1(ry);
+            rz = wave_ror1(rz);
+        }
+        const int slot = tile * 64 + lane;
+        jacc[0][slot] += rx;
+        jacc[1][slot] += ry;
+        jacc[2][slot] += rz;
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < B; k += 64 * W) {
+        const size_t o = (size_t)P * npad + (size_t)Q * B + k;
+        part[o] = jacc[0][k];
+        part[plane + o] = jacc[1][k];
+        part[2 * plane + o] = jacc[2][k];
+    }
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+        const size_t o = (size_t)Q * npad + (size_t)P * B + u * 64 * W + threadIdx.x;
+        part[o] = acc.x[u];
+        part[plane + o] = acc.y[u];
+        part[2 * plane + o] = acc.z[u];
+    }
+}
+
+// Piece k (of A) of block P's diagonal: its 64*W threads take bodies
+// P*B + k*64*W + threadIdx.x against block P's bodies, a plain sum in j order,
+// into slot P. Lanes past n compute on a zero body and store into padding
+// that no reduction reads. CONST_VIEW reads the j bodies through the constant
+// address space (pointers that come from memory rather than kernel arguments).
+template <int B, bool NEWTON, bool CONST_VIEW>
+__device__ __forceinline__ void sym_diag_piece(const float4* __restrict__ pm, int n, int npad, float eps2,
+                                               float* __restrict__ part, int P, int k, int threads)
+{
+    if (P * B + k * threads >= n) return;  // a piece made only of padding (workgroup-uniform)
+    const int i = P * B + k * threads + (int)threadIdx.x;
+    const float4 q = i < n ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    Lanes<1> me, acc;
+    me.x[0] = q.x;
+    me.y[0] = q.y;
+    me.z[0] = q.z;
+    zero<1>(acc);
+    const int j0 = P * B;
+    if constexpr (CONST_VIEW) sweep_const<1, NEWTON>(pm + j0, min(n - j0, B), eps2, me, acc);
+    else sweep_sgpr<1, NEWTON>(pm + j0, min(n - j0, B), eps2, me, acc);
+    const size_t plane = (size_t)(npad / B) * npad;
+    const size_t o = (size_t)P * npad + i;
+    part[o] = acc.x[0];
+    part[plane + o] = acc.y[0];
+    part[2 * plane + o] = acc.z[0];
+}
+
+}  // namespace nbody

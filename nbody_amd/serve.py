@@ -27,6 +27,11 @@ def poisson_trace(count, rate_hz, n_lo, n_hi, steps_lo, steps_hi, seed, eps2=0.0
     return reqs
 
 
+def request_bytes(n):
+    """Device bytes a resident request of n bodies holds (what the budget counts)."""
+    return int(_native.lib().nbody_hip_request_bytes(int(n)))
+
+
 class Server:
     def __init__(self, budget_bytes=200 * 10 ** 9, max_active=64, dt=0.016, damping=0.95, keep_states=False):
         self.L = _native.lib()

@@ -87,9 +87,12 @@ def test_served_equals_ensemble_batched():
 def test_budget_rejects_and_bounds():
     reqs = (_native.Request * 3)()
     reqs[0] = _native.Request(0, 1000, 2, 1, 0, 0.01, 0.0)
-    reqs[1] = _native.Request(1, 5000, 2, 2, 1, 0.01, 0.0)   # 5000 * 48 B > budget: rejected
+    reqs[1] = _native.Request(1, 5000, 2, 2, 1, 0.01, 0.0)   # more than the budget: rejected
     reqs[2] = _native.Request(2, 2000, 3, 3, 0, 0.01, 0.001)
-    srv = serve.Server(budget_bytes=2000 * 48, max_active=4)
+    budget = serve.request_bytes(2000)
+    assert serve.request_bytes(1000) + budget > budget >= serve.request_bytes(1000)
+    assert serve.request_bytes(5000) > budget
+    srv = serve.Server(budget_bytes=budget, max_active=4)
     try:
         res, _, work = srv.run(reqs)
     finally:

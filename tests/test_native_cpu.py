@@ -142,3 +142,14 @@ def test_poisson_trace_statistics():
     assert 0.45 < ics_kind.mean() < 0.55
     again = serve.poisson_trace(4000, 200.0, 1024, 131072, 10, 200, seed=9)
     assert all(a.N == b.N and a.seed == b.seed and a.arrival_s == b.arrival_s for a, b in zip(reqs, again))
+
+
+def test_request_bytes_closed_form():
+    from nbody_amd import serve
+    # 48 B per body, plus 3 float planes of ceil(N/512) slots x ceil(N/512)*512 bodies
+    for n in (1, 512, 513, 1000, 16384, 2 ** 18):
+        nb = -(-n // 512)
+        assert serve.request_bytes(n) == 48 * n + 12 * nb * nb * 512, n
+    # beyond 2^18 bodies a single compensated slot per body
+    n = 2 ** 18 + 1
+    assert serve.request_bytes(n) == 48 * n + 12 * n
