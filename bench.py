@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--ipl", type=int, default=0, help="pin i bodies per lane (skips the autotuner)")
     ap.add_argument("--jslice", type=int, default=0, help="pin the j slice length (skips the autotuner)")
     ap.add_argument("--compensated", type=int, default=-1, choices=[-1, 0, 1])
+    ap.add_argument("--waves", type=int, default=0, help="pin the symmetric plan's waves per SIMD (skips the autotuner)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration for --pmc runs)")
     ap.add_argument("--workload", default="single", choices=["single", "ensemble", "served", "plummer"],
@@ -337,9 +338,9 @@ def main():
     variant = {"auto": _native.AUTO, "sgpr": _native.SGPR, "lds": _native.LDS, "dpp": _native.DPP,
                "aos": _native.AOS, "symmetric": _native.SYMMETRIC}[args.variant]
     opt = _native.Options.make(variant=variant, i_per_lane=args.ipl, jslice=args.jslice,
-                               compensated=args.compensated)
+                               compensated=args.compensated, waves_per_simd=args.waves)
     P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
-    pinned = args.ipl > 0 or args.jslice > 0
+    pinned = args.ipl > 0 or args.jslice > 0 or args.waves > 0
 
     if not args.no_autotune and not pinned and variant != _native.AOS:
         _native.check_ms(L.nbody_hip_autotune(P(force), P(pm), eps2, n, variant), "autotune")
@@ -403,7 +404,8 @@ def main():
             "n_bodies": n,
             "interactions_per_step": int(inter),
             "plan": {"variant": _native.VARIANT_NAMES[plan.variant], "i_per_lane": plan.i_per_lane,
-                     "jslice": plan.jslice, "compensated": bool(plan.compensated), "newton": bool(plan.newton)},
+                     "jslice": plan.jslice, "compensated": bool(plan.compensated), "newton": bool(plan.newton),
+                     "waves_per_simd": plan.waves_per_simd},
             "vs_baseline_ref": "GPU_Shared, RTX 2080 Ti, clang build: 501.5 x10^9 int/s (README.md:230-233)",
             "device": f"{arch.value.decode()} {cus.value} CUs {clk.value // 1000} MHz",
         },

@@ -92,6 +92,7 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
         }
         p.nslices = (int)((N + p.jslice - 1) / p.jslice);
         p.comp = true;
+        p.occ = opt ? opt->waves_per_simd : 0;
         return p;
     }
     if (variant == VAR_AOS) {
@@ -148,6 +149,10 @@ int validate_plan(const ForcePlan& p, size_t N)
                       "(got %d, %d)", p.ipl, p.jslice);
             return E_ARG;
         }
+        if (p.occ < 0 || p.occ > 8) {
+            set_error("waves_per_simd must be 0..8 (got %d)", p.occ);
+            return E_ARG;
+        }
         if (p.nslices > kMaxSymBlocks) {
             set_error("N = %zu needs %d symmetric blocks of %d (at most %d)", N, p.nslices, p.jslice, kMaxSymBlocks);
             return E_UNSUPPORTED;
@@ -170,7 +175,7 @@ int make_plan(DeviceState& d, size_t N, size_t n, const nbody_hip_options* opt, 
         return E_ARG;
     }
     ForcePlan p = default_plan(variant, N, n, d.cus, opt);
-    const bool explicit_shape = opt && (opt->i_per_lane > 0 || opt->jslice > 0);
+    const bool explicit_shape = opt && (opt->i_per_lane > 0 || opt->jslice > 0 || opt->waves_per_simd > 0);
     if (!explicit_shape) {
         auto it = d.tuned.find(tune_key(opt, N, n));
         if (it != d.tuned.end()) p = it->second;
@@ -470,6 +475,7 @@ int nbody_hip_plan(size_t N, size_t n, const nbody_hip_options* opt, nbody_hip_o
     out->jslice = p.jslice;
     out->compensated = p.comp ? 1 : 0;
     out->newton = p.newton ? 1 : 0;
+    out->waves_per_simd = p.occ;
     return OK;
 }
 
@@ -499,7 +505,14 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
                 o.i_per_lane = sh[0];
                 o.jslice = sh[1];
                 ForcePlan c = default_plan(fam, N, N, d->cus, &o);
-                if (c.nslices <= kMaxSymBlocks) cands.push_back(c);
+                if (c.nslices > kMaxSymBlocks) continue;
+                cands.push_back(c);
+                // the same shape with one and two fewer resident waves per SIMD
+                const int natural = symmetric_natural_occupancy(c);
+                for (int less = 1; less <= 2 && natural - less >= 3; ++less) {
+                    c.occ = natural - less;
+                    cands.push_back(c);
+                }
             }
         } else {
             for (int ipl : {1, 2, 4}) {

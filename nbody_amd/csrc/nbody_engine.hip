@@ -268,6 +268,7 @@ int nbody_hip_system_plan(const nbody_hip_system* s, nbody_hip_options* plan)
     plan->jslice = s->plan.jslice;
     plan->compensated = s->plan.comp ? 1 : 0;
     plan->newton = s->plan.newton ? 1 : 0;
+    plan->waves_per_simd = s->plan.occ;
     return OK;
 }
 

@@ -28,6 +28,7 @@ struct ForcePlan {
     int nslices = 1;    // ceil(n / jslice)
     bool comp = false;  // TwoSum-compensated accumulation
     bool newton = false;
+    int occ = 0;        // symmetric: waves per SIMD cap (0 = register-limited occupancy)
 };
 
 struct ForceParams {
@@ -45,6 +46,10 @@ struct ForceParams {
 
 // Whether (i per lane, block size) is a compiled symmetric shape.
 bool symmetric_shape_ok(int ipl, int block);
+
+// Waves per SIMD the plan's symmetric kernel reaches when its registers and
+// static LDS alone limit it (0 for other variants or on error).
+int symmetric_natural_occupancy(const ForcePlan& plan);
 
 // Bytes of slice-partial scratch a plan needs for ni bodies (0 when direct).
 size_t plan_scratch_bytes(const ForcePlan& plan, int ni);

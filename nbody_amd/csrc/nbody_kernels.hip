@@ -16,6 +16,11 @@
 #include "nbody_kernels.h"
 #include "nbody_sym.h"
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 namespace nbody {
 
 namespace {
@@ -447,13 +452,53 @@ hipError_t go_family(const ForcePlan& plan, dim3 g, hipStream_t s, const ForcePa
     return hipGetLastError();
 }
 
+// Resident workgroups per CU of `kernel` with `dyn` bytes of dynamic LDS.
+int blocks_per_cu(const void* kernel, int threads, size_t dyn)
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, threads, dyn) != hipSuccess) return 0;
+    return blocks;
+}
+
+// Smallest extra (dynamic) LDS per workgroup that keeps at most `waves`
+// waves of the kernel on each SIMD (4 SIMDs per CU); 0 when the registers
+// already keep it there. Cached per (kernel, waves).
+size_t lds_for_occupancy(const void* kernel, int threads, int waves)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, size_t> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(kernel, waves);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    const int target = std::max(1, 4 * waves / (threads / 64));
+    size_t dyn = 0;
+    while (dyn < 160 * 1024 && blocks_per_cu(kernel, threads, dyn) > target) dyn += 256;
+    cache[key] = dyn;
+    return dyn;
+}
+
+template <int A, int W, bool NEWTON>
+const void* sym_kernel()
+{
+    return reinterpret_cast<const void*>(&k_force_sym<A, W, NEWTON>);
+}
+
 template <int A, int W>
 hipError_t go_sym(const ForcePlan& plan, const SymParams& sp, hipStream_t s)
 {
     const int tasks = sp.nb * (sp.nb - 1) / 2 + sp.nb * A;
-    if (plan.newton) k_force_sym<A, W, true><<<tasks, 64 * W, 0, s>>>(sp);
-    else k_force_sym<A, W, false><<<tasks, 64 * W, 0, s>>>(sp);
+    const void* k = plan.newton ? sym_kernel<A, W, true>() : sym_kernel<A, W, false>();
+    const size_t dyn = plan.occ > 0 ? lds_for_occupancy(k, 64 * W, plan.occ) : 0;
+    if (plan.newton) k_force_sym<A, W, true><<<tasks, 64 * W, dyn, s>>>(sp);
+    else k_force_sym<A, W, false><<<tasks, 64 * W, dyn, s>>>(sp);
     return hipGetLastError();
+}
+
+template <int A, int W>
+int natural_occupancy(bool newton)
+{
+    return blocks_per_cu(newton ? sym_kernel<A, W, true>() : sym_kernel<A, W, false>(), 64 * W, 0) * W / 4;
 }
 
 }  // namespace
@@ -463,6 +508,18 @@ bool symmetric_shape_ok(int ipl, int block)
     const int w = block / (64 * ipl);
     if (w * 64 * ipl != block) return false;
     return (ipl == 2 && w == 4) || (ipl == 4 && (w == 1 || w == 2 || w == 4)) || (ipl == 8 && (w == 1 || w == 2 || w == 4));
+}
+
+int symmetric_natural_occupancy(const ForcePlan& plan)
+{
+    if (plan.variant != VAR_SYMMETRIC || !symmetric_shape_ok(plan.ipl, plan.jslice)) return 0;
+    const int w = plan.jslice / (64 * plan.ipl);
+    if (plan.ipl == 2) return natural_occupancy<2, 4>(plan.newton);
+    if (plan.ipl == 4)
+        return w == 1 ? natural_occupancy<4, 1>(plan.newton) : w == 2 ? natural_occupancy<4, 2>(plan.newton)
+                                                                      : natural_occupancy<4, 4>(plan.newton);
+    return w == 1 ? natural_occupancy<8, 1>(plan.newton) : w == 2 ? natural_occupancy<8, 2>(plan.newton)
+                                                                  : natural_occupancy<8, 4>(plan.newton);
 }
 
 size_t plan_scratch_bytes(const ForcePlan& plan, int ni)

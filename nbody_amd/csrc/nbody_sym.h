@@ -45,6 +45,71 @@ __device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
     P = t - q * (q - 1) / 2;
 }
 
+// One rotation step of a block pair with 8 i bodies per lane, in assembly so
+// that its schedule is fixed: each pair's instructions run as one dependent
+// chain (differences, d2, v_rsq_f32, the products, the reaction FMAs, then the
+// FMAs into the i accumulators), the pairs one after another. Measured on
+// gfx950 this order issues about 10% more VALU instructions per clock than
+// any interleaving of independent pairs (the compiler's schedule interleaves
+// them); the bank of each register hardly matters, so the compiler allocates
+// them. One s_nop follows each v_rsq_f32 (the wait state gfx950 requires
+// before a VALU reads a transcendental result), and the DPP rotations come
+// four or more instructions after the last write of what they move. The
+// arithmetic and its order are exactly those of the generic loop in sym_pair
+// (same operations per accumulator, u in increasing order), so results are
+// bit-identical to it.
+#define NBODY_SYM_PAIR(u)                                                                              \
+    "v_sub_f32 %[dx], %[xj], %[x" #u "]\n"                                                             \
+    "v_sub_f32 %[dy], %[yj], %[y" #u "]\n"                                                             \
+    "v_sub_f32 %[dz], %[zj], %[z" #u "]\n"                                                             \
+    "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                                           \
+    "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                                  \
+    "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                                  \
+    "v_rsq_f32 %[s], %[s]\n"                                                                           \
+    "s_nop 0\n"                                                                                        \
+    "v_mul_f32 %[t], %[s], %[s]\n"                                                                     \
+    "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
+    "v_mul_f32 %[s], %[mj], %[t]\n"                                                                    \
+    "v_mul_f32 %[t], %[n" #u "], %[t]\n"                                                               \
+    "v_fmac_f32 %[rx], %[dx], %[t]\n"                                                                  \
+    "v_fmac_f32 %[ry], %[dy], %[t]\n"                                                                  \
+    "v_fmac_f32 %[rz], %[dz], %[t]\n"                                                                  \
+    "v_fmac_f32 %[ax" #u "], %[dx], %[s]\n"                                                            \
+    "v_fmac_f32 %[ay" #u "], %[dy], %[s]\n"                                                            \
+    "v_fmac_f32 %[az" #u "], %[dz], %[s]\n"
+
+#define NBODY_ROR1(r) "v_mov_b32_dpp %[" #r "], %[" #r "] wave_ror:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+
+__device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
+                                          Lanes<8>& acc, const Lanes<8>& me, const float (&nm)[8], float eps2)
+{
+    float dx, dy, dz, s, t;
+    asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
+                 NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR(7)
+                 NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj)
+                 NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+                 : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry),
+                   [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                   [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
+                   [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
+                   [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
+                   [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
+                   [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
+                   [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s),
+                   [t] "=&v"(t)
+                 : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
+                   [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
+                   [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
+                   [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
+                   [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
+                   [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]),
+                   [n0] "v"(nm[0]), [n1] "v"(nm[1]), [n2] "v"(nm[2]), [n3] "v"(nm[3]),
+                   [n4] "v"(nm[4]), [n5] "v"(nm[5]), [n6] "v"(nm[6]), [n7] "v"(nm[7]), [eps] "s"(eps2));
+}
+
+#undef NBODY_SYM_PAIR
+#undef NBODY_ROR1
+
 // Block pair (P, Q), P < Q. `jacc` is the workgroup's LDS reaction buffer.
 // P is never the last (possibly ragged) block, so every i lane holds a real
 // body; j bodies past n enter with zero mass.
@@ -74,11 +139,15 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
     __syncthreads();
     for (int ph = 0; ph < T; ++ph) {
         const int tile = (ph + wave * A) % T;
-        const int jidx = Q * B + tile * 64 + lane;
-        const float4 b = jidx < n ? pm[jidx] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int jbase = Q * B + tile * 64;  // wave-uniform
+        const float4 b = lane < n - jbase ? pm[jbase + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
         float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
         float rx = 0.f, ry = 0.f, rz = 0.f;
         for (int k = 0; k < 64; ++k) {
+            if constexpr (A == 8 && !NEWTON) {
+                sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
+                continue;
+            }
 #pragma unroll
             for (int u = 0; u < A; ++u) {
                 const float dx = xj - me.x[u];

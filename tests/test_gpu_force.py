@@ -127,6 +127,26 @@ def test_symmetric_rejects_i_ranges():
     assert rc != 0 and "block" in _native.last_error()
 
 
+def test_symmetric_waves_per_simd_leaves_results_alone():
+    """Capping the symmetric kernel's resident waves (extra LDS per workgroup)
+    changes only the schedule: the forces stay bit-identical, the plan reports
+    the cap, and caps outside 0..8 are rejected."""
+    pm, _ = bodies(7000, seed=6)
+    d = dev(pm)
+    base = run_force(d, EPS2, variant=_native.SYMMETRIC, i_per_lane=8, jslice=512)
+    for waves in (6, 4, 2, 1):
+        f = run_force(d, EPS2, variant=_native.SYMMETRIC, i_per_lane=8, jslice=512, waves_per_simd=waves)
+        assert np.array_equal(f.view(np.uint32), base.view(np.uint32)), waves
+    o = _native.Options.make(variant=_native.SYMMETRIC, waves_per_simd=5)
+    plan = _native.Options()
+    _native.check(_native.lib().nbody_hip_plan(7000, 7000, ctypes.byref(o), ctypes.byref(plan)), "plan")
+    assert plan.waves_per_simd == 5
+    o = _native.Options.make(variant=_native.SYMMETRIC, waves_per_simd=9)
+    f = force_buffer(7000)
+    rc = _native.lib().nbody_hip_force(ptr(f), ptr(d), EPS2, 7000, 0, 7000, ctypes.byref(o), None)
+    assert rc != 0 and "waves_per_simd" in _native.last_error()
+
+
 @pytest.mark.parametrize("variant", [_native.SGPR, _native.LDS, _native.DPP, _native.SYMMETRIC])
 def test_deterministic(variant):
     pm, _ = bodies(20000, seed=4)
