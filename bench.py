@@ -424,6 +424,20 @@ def main():
         "accuracy": {"max_rel": max_rel, "rms_rel": rms_rel,
                      "vs": "FP64 direct sum (GPU diagnostic), per-body |F-F64|/|F64|"},
     }
+    # the host-pointer table entry (reference AOS contract): H2D of posMass, the
+    # force kernel and D2H of the forces, the PCIe-inclusive rate of one evaluation
+    host_f = np.zeros((n, 4), np.float32)
+    host_pm = np.ascontiguousarray(pm.cpu().numpy())
+    hms = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        _native.check_ms(L.ComputeGravitation_HIP_Host(host_f.ctypes.data, host_pm.ctypes.data, eps2, n),
+                         "ComputeGravitation_HIP_Host")
+        hms.append((time.perf_counter() - t0) * 1e3)
+    host_ms = sorted(hms[1:])[len(hms[1:]) // 2]
+    out["host_entry"] = {"ms_per_call": round(host_ms, 4), "value": round(inter / (host_ms * 1e-3) / 1e9, 2),
+                         "unit": "10^9 interactions/s",
+                         "what": "ComputeGravitation_HIP_Host on host buffers: H2D posMass + force + D2H, median of 5"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pm_h, vm_h, np.float32(eps2), dt, damping, args.cpu_budget_s)
     print(json.dumps(out))
