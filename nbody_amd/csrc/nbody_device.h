@@ -57,6 +57,52 @@ __device__ __forceinline__ void body_body(float xj, float yj, float zj, float mj
     az = __builtin_fmaf(dz, s, az);
 }
 
+// The same interaction as one fixed dependent chain in assembly: gfx950
+// issues a chain whose instructions consume each other's results back to back
+// at a measurably higher rate than the compiler's interleaving of independent
+// interactions. Same operations and rounding as body_body<false> (bit-
+// identical); one s_nop after v_rsq_f32 is the wait state gfx950 requires
+// before a VALU reads a transcendental result. JC is the register class of
+// the j body: "v" (from LDS or a DPP tile) or "s" (from scalar loads).
+#define NBODY_BODY_CHAIN(NAME, JC)                                                                     \
+    __device__ __forceinline__ void NAME(float xj, float yj, float zj, float mj, float xi, float yi,   \
+                                         float zi, float eps2, float& ax, float& ay, float& az)        \
+    {                                                                                                  \
+        float dx, dy, dz, s, t;                                                                        \
+        asm volatile("v_sub_f32 %[dx], %[xj], %[xi]\n"                                                 \
+                     "v_sub_f32 %[dy], %[yj], %[yi]\n"                                                 \
+                     "v_sub_f32 %[dz], %[zj], %[zi]\n"                                                 \
+                     "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                          \
+                     "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                 \
+                     "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                 \
+                     "v_rsq_f32 %[s], %[s]\n"                                                          \
+                     "s_nop 0\n"                                                                       \
+                     "v_mul_f32 %[t], %[mj], %[s]\n"                                                   \
+                     "v_mul_f32 %[s], %[s], %[s]\n"                                                    \
+                     "v_mul_f32 %[s], %[t], %[s]\n"                                                    \
+                     "v_fmac_f32 %[ax], %[dx], %[s]\n"                                                 \
+                     "v_fmac_f32 %[ay], %[dy], %[s]\n"                                                 \
+                     "v_fmac_f32 %[az], %[dz], %[s]\n"                                                 \
+                     : [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [dx] "=&v"(dx), [dy] "=&v"(dy),     \
+                       [dz] "=&v"(dz), [s] "=&v"(s), [t] "=&v"(t)                                      \
+                     : [xj] JC(xj), [yj] JC(yj), [zj] JC(zj), [mj] JC(mj), [xi] "v"(xi), [yi] "v"(yi),  \
+                       [zi] "v"(zi), [eps] "s"(eps2));                                                 \
+    }
+NBODY_BODY_CHAIN(body_body_chain_v, "v")
+NBODY_BODY_CHAIN(body_body_chain_s, "s")
+#undef NBODY_BODY_CHAIN
+
+// body_body for the kernels: the assembly chain when no Newton step is asked
+// for, with the j body in VGPRs (SGPR_J false) or SGPRs (SGPR_J true).
+template <bool NEWTON, bool SGPR_J>
+__device__ __forceinline__ void interact(float xj, float yj, float zj, float mj, float xi, float yi, float zi,
+                                         float eps2, float& ax, float& ay, float& az)
+{
+    if constexpr (NEWTON) body_body<true>(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
+    else if constexpr (SGPR_J) body_body_chain_s(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
+    else body_body_chain_v(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
+}
+
 // DPP row_newbcast:K (gfx90a and later): every lane of each 16-lane row reads
 // lane K of its row. The combine pass folds it into the consuming VOP2
 // (v_sub_f32_dpp / v_mul_f32_dpp), so the broadcast costs no extra VALU op.
@@ -100,18 +146,42 @@ __device__ __forceinline__ void fold(Lanes<I>& hi, Lanes<I>& lo, const Lanes<I>&
 // (four bodies per load) and the VALU ops take x_j, y_j, z_j, m_j as SGPR
 // operands, so no instruction is spent on the broadcast; on gfx950 a VALU op
 // that reads an SGPR issues at about 60% of the all-VGPR rate, though.
+// One sweep over j bodies in groups of G: the G loads are issued before the
+// G x I interaction chains that consume them (the chains are assembly the
+// compiler neither reorders nor unrolls around), then the remainder.
+template <int I, bool NEWTON, bool SGPR_J, int G, class Load>
+__device__ __forceinline__ void sweep_grouped(const Load& load, int len, float eps2, const Lanes<I>& me,
+                                              Lanes<I>& acc)
+{
+    int j = 0;
+    for (; j + G <= len; j += G) {
+        float4 b[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) b[q] = load(j + q);
+#pragma unroll
+        for (int q = 0; q < G; ++q)
+#pragma unroll
+            for (int u = 0; u < I; ++u)
+                interact<NEWTON, SGPR_J>(b[q].x, b[q].y, b[q].z, b[q].w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u],
+                                         acc.y[u], acc.z[u]);
+    }
+    for (; j < len; ++j) {
+        const float4 b = load(j);
+#pragma unroll
+        for (int u = 0; u < I; ++u)
+            interact<NEWTON, SGPR_J>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
+                                     acc.z[u]);
+    }
+}
+
+// j bodies read with wave-uniform addresses: the compiler emits s_load_dwordx16
+// (four bodies per load) and the VALU ops take x_j, y_j, z_j, m_j as SGPR
+// operands, so no instruction is spent on the broadcast.
 template <int I, bool NEWTON>
 __device__ __forceinline__ void sweep_sgpr(const float4* __restrict__ src, int len, float eps2,
                                            const Lanes<I>& me, Lanes<I>& acc)
 {
-#pragma unroll 8
-    for (int j = 0; j < len; ++j) {
-        const float4 b = src[j];
-#pragma unroll
-        for (int u = 0; u < I; ++u)
-            body_body<NEWTON>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
-                              acc.z[u]);
-    }
+    sweep_grouped<I, NEWTON, true, 8>([src](int k) { return src[k]; }, len, eps2, me, acc);
 }
 
 // Read-only view of body records for kernels whose body pointers are not
@@ -125,13 +195,9 @@ __device__ __forceinline__ void sweep_const(const float4* src4, int len, float e
                                             Lanes<I>& acc)
 {
     const const_body_float* src = (const const_body_float*)(const void*)src4;
-#pragma unroll 8
-    for (int j = 0; j < len; ++j) {
-        const float xj = src[4 * j + 0], yj = src[4 * j + 1], zj = src[4 * j + 2], mj = src[4 * j + 3];
-#pragma unroll
-        for (int u = 0; u < I; ++u)
-            body_body<NEWTON>(xj, yj, zj, mj, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u], acc.z[u]);
-    }
+    sweep_grouped<I, NEWTON, true, 8>(
+        [src](int k) { return make_float4(src[4 * k + 0], src[4 * k + 1], src[4 * k + 2], src[4 * k + 3]); }, len,
+        eps2, me, acc);
 }
 
 }  // namespace nbody

@@ -133,14 +133,8 @@ __global__ __launch_bounds__(kBlock) void k_force_lds(ForceParams p)
         Lanes<I> part;
         zero<I>(part);
         Lanes<I>& acc = COMP ? part : hi;
-#pragma unroll 8
-        for (int q = 0; q < cnt; ++q) {
-            const float4 b = tile[buf][q];
-#pragma unroll
-            for (int u = 0; u < I; ++u)
-                body_body<NEWTON>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], p.eps2, acc.x[u],
-                                  acc.y[u], acc.z[u]);
-        }
+        const float4* t4 = tile[buf];
+        sweep_grouped<I, NEWTON, false, 4>([t4](int q) { return t4[q]; }, cnt, p.eps2, me, acc);
         if constexpr (COMP) fold<I>(hi, lo, part);
     }
     store<I>(p, hi, COMP ? &lo : nullptr);

@@ -153,3 +153,19 @@ def test_request_bytes_closed_form():
     # beyond 2^18 bodies a single compensated slot per body
     n = 2 ** 18 + 1
     assert serve.request_bytes(n) == 48 * n + 12 * n
+
+
+def test_size_and_range_limits_fail_before_any_device_call():
+    """Argument checks run before the library touches a device: N above the
+    2^28 cap, N = 0, and i-ranges outside [0, N) return the failure value
+    with a message (the pointers are never dereferenced)."""
+    L = _native.lib()
+    fake = ctypes.c_void_p(16)
+    assert L.ComputeGravitation_HIP_LDS(fake, fake, 0.01, 2 ** 28 + 1) == 0.0
+    assert "outside" in _native.last_error()
+    assert L.ComputeGravitation_HIP(fake, fake, 0.01, 0) == 0.0
+    assert "N = 0" in _native.last_error()
+    o = _native.Options.make()
+    assert L.nbody_hip_force(fake, fake, 0.01, 100, 90, 11, ctypes.byref(o), None) != 0
+    assert "i range" in _native.last_error()
+    assert L.nbody_hip_force(fake, fake, 0.01, 100, 101, 0, ctypes.byref(o), None) != 0
