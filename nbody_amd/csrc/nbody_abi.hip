@@ -536,17 +536,24 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
     for (const ForcePlan& c : cands) {
         const size_t bytes = plan_scratch_bytes(c, (int)N);
         if (bytes && d->partial.reserve(bytes) != hipSuccess) continue;
+        // one untimed launch, one to size the batches (about 2 ms of launches
+        // each, so short kernels are not timed at the resolution of a single
+        // launch), then the best per-launch average of three batches
         float t = 0.0f;
-        for (int rep = 0; rep < 3; ++rep) {
+        int batch = 1;
+        for (int rep = 0; rep < 5; ++rep) {
             if (hipEventRecord(d->ev0, nullptr) != hipSuccess) return 0.0f;
-            if (run_force(*d, c, force, posMass, softeningSquared, N, 0, N, nullptr)) return 0.0f;
+            for (int k = 0; k < batch; ++k)
+                if (run_force(*d, c, force, posMass, softeningSquared, N, 0, N, nullptr)) return 0.0f;
             float ms = 0.0f;
             if (hipEventRecord(d->ev1, nullptr) != hipSuccess || hipEventSynchronize(d->ev1) != hipSuccess ||
                 hipEventElapsedTime(&ms, d->ev0, d->ev1) != hipSuccess) {
                 set_error("autotune timing failed");
                 return 0.0f;
             }
-            if (rep > 0 && (t == 0.0f || ms < t)) t = ms;
+            ms /= (float)batch;
+            if (rep == 1) batch = std::max(1, std::min(64, (int)(2.0f / std::max(ms, 1e-3f))));
+            if (rep >= 2 && (t == 0.0f || ms < t)) t = ms;
         }
         if (t > 0.0f && (best == 0.0f || t < best)) {
             best = t;
