@@ -140,45 +140,52 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
     for (int ph = 0; ph < T; ++ph) {
         const int tile = (ph + wave * A) % T;
         const int jbase = Q * B + tile * 64;  // wave-uniform
-        const float4 b = lane < n - jbase ? pm[jbase + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
-        float rx = 0.f, ry = 0.f, rz = 0.f;
-        for (int k = 0; k < 64; ++k) {
-            if constexpr (A == 8 && !NEWTON) {
-                sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
-                continue;
-            }
+        // a tile past the end of a ragged last block holds only zero-mass
+        // padding, which would add exact zeros to the i side: one-wave
+        // workgroups skip its work (the barrier is still met; with more waves
+        // the extra branch costs registers, and those shapes serve large
+        // systems where ragged blocks are rare)
+        if (W > 1 || jbase < n) {
+            const float4 b = lane < n - jbase ? pm[jbase + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+            float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
+            float rx = 0.f, ry = 0.f, rz = 0.f;
+            for (int k = 0; k < 64; ++k) {
+                if constexpr (A == 8 && !NEWTON) {
+                    sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
+                    continue;
+                }
 #pragma unroll
-            for (int u = 0; u < A; ++u) {
-                const float dx = xj - me.x[u];
-                const float dy = yj - me.y[u];
-                const float dz = zj - me.z[u];
-                float d2 = __builtin_fmaf(dx, dx, eps2);
-                d2 = __builtin_fmaf(dy, dy, d2);
-                d2 = __builtin_fmaf(dz, dz, d2);
-                const float inv = rsqrt_f32<NEWTON>(d2);
-                const float inv3 = (inv * inv) * inv;
-                const float si = mj * inv3;
-                const float sj = nm[u] * inv3;
-                acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
-                acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
-                acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
-                rx = __builtin_fmaf(dx, sj, rx);
-                ry = __builtin_fmaf(dy, sj, ry);
-                rz = __builtin_fmaf(dz, sj, rz);
+                for (int u = 0; u < A; ++u) {
+                    const float dx = xj - me.x[u];
+                    const float dy = yj - me.y[u];
+                    const float dz = zj - me.z[u];
+                    float d2 = __builtin_fmaf(dx, dx, eps2);
+                    d2 = __builtin_fmaf(dy, dy, d2);
+                    d2 = __builtin_fmaf(dz, dz, d2);
+                    const float inv = rsqrt_f32<NEWTON>(d2);
+                    const float inv3 = (inv * inv) * inv;
+                    const float si = mj * inv3;
+                    const float sj = nm[u] * inv3;
+                    acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
+                    acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
+                    acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
+                    rx = __builtin_fmaf(dx, sj, rx);
+                    ry = __builtin_fmaf(dy, sj, ry);
+                    rz = __builtin_fmaf(dz, sj, rz);
+                }
+                xj = wave_ror1(xj);
+                yj = wave_ror1(yj);
+                zj = wave_ror1(zj);
+                mj = wave_ror1(mj);
+                rx = wave_ror1(rx);
+                ry = wave_ror1(ry);
+                rz = wave_ror1(rz);
             }
-            xj = wave_ror1(xj);
-            yj = wave_ror1(yj);
-            zj = wave_ror1(zj);
-            mj = wave_ror1(mj);
-            rx = wave_ror1(rx);
-            ry = wave_ror1(ry);
-            rz = wave_ror1(rz);
+            const int slot = tile * 64 + lane;
+            jacc[0][slot] += rx;
+            jacc[1][slot] += ry;
+            jacc[2][slot] += rz;
         }
-        const int slot = tile * 64 + lane;
-        jacc[0][slot] += rx;
-        jacc[1][slot] += ry;
-        jacc[2][slot] += rz;
         __syncthreads();
     }
     for (int k = threadIdx.x; k < B; k += 64 * W) {
