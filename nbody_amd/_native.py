@@ -127,6 +127,15 @@ def lib():
         raise NativeLibraryMissing(
             f"{LIB_PATH} is not built; run `python -m nbody_amd.build` (needs hipcc). "
             "nbody_amd has no CPU fallback.")
+    # PyTorch, when installed, is imported first: it loads its own HIP runtime,
+    # which carries the same soname (libamdhip64.so.7) as the one this library
+    # links, so the loader binds the library to it and the process holds one
+    # HIP runtime. Loaded the other way round, torch would bring a second
+    # runtime into the process, and its device initialisation can then fail.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)

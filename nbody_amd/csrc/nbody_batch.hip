@@ -16,6 +16,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kChunk = 128;                  // j bodies between TwoSum folds of a ROW_COMP task
 constexpr int kPairA = kBatchBlock / kWave;  // bodies per lane of a PAIR task
+constexpr int kBodyTaskBodies = 256;         // bodies per BodyTask (one thread each, per component)
 
 static_assert(kBatchBlock % kWave == 0 && kPairA == 8, "PAIR tasks are compiled for 8 bodies per lane");
 
@@ -67,46 +68,51 @@ __global__ __launch_bounds__(kWave) void k_force_batch(const ForceTask* __restri
     }
 }
 
-// One wave per task. The slot sum is k_reduce_slots' (slot order, TwoSum,
-// rounded once) and the update is k_integrate's, both in nbody_kernels.hip.
-__global__ __launch_bounds__(kWave) void k_reduce_integrate_batch(const BodyTask* __restrict__ tasks, float dt,
-                                                                  float damping, unsigned* __restrict__ counter)
+// 3 x kBodyTaskBodies threads per task: the threads of component c add that
+// component of the task's bodies over their slots (eight slots in flight,
+// each slot row read as one contiguous 1 KiB run), then the first
+// kBodyTaskBodies threads integrate. The slot sum is k_reduce_slots' (slot
+// order, TwoSum, rounded once) and the update is k_integrate's, both in
+// nbody_kernels.hip.
+__global__ __launch_bounds__(3 * kBodyTaskBodies) void k_reduce_integrate_batch(const BodyTask* __restrict__ tasks,
+                                                                                float dt, float damping,
+                                                                                unsigned* __restrict__ counter)
 {
     constexpr int G = 8;
+    __shared__ float sum[3][kBodyTaskBodies];
     if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
     const BodyTask task = tasks[blockIdx.x];
-    if ((int)threadIdx.x >= task.ni) return;
-    const int i = task.i0 + threadIdx.x;
-    const size_t plane = (size_t)task.nb * task.npad;
-    const float* src = task.part + i;
-    float s[3] = {0.f, 0.f, 0.f}, e[3] = {0.f, 0.f, 0.f};
-    int k = 0;
-    for (; k + G <= task.nb; k += G) {
-        float v[3][G];
+    const int lane = threadIdx.x % kBodyTaskBodies;
+    const int c = threadIdx.x / kBodyTaskBodies;  // wave-uniform component
+    if (lane < task.ni) {
+        const float* src = task.part + (size_t)c * task.nb * task.npad + task.i0 + lane;
+        float s = 0.f, e = 0.f;
+        int k = 0;
+        for (; k + G <= task.nb; k += G) {
+            float v[G];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int g = 0; g < G; ++g) v[c][g] = src[c * plane + (size_t)(k + g) * task.npad];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
+            for (int g = 0; g < G; ++g) v[g] = src[(size_t)(k + g) * task.npad];
 #pragma unroll
             for (int g = 0; g < G; ++g) {
                 float err;
-                s[c] = two_sum(s[c], v[c][g], err);
-                e[c] += err;
+                s = two_sum(s, v[g], err);
+                e += err;
             }
-    }
-    for (; k < task.nb; ++k)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float err;
-            s[c] = two_sum(s[c], src[c * plane + (size_t)k * task.npad], err);
-            e[c] += err;
         }
+        for (; k < task.nb; ++k) {
+            float err;
+            s = two_sum(s, src[(size_t)k * task.npad], err);
+            e += err;
+        }
+        sum[c][lane] = s + e;
+    }
+    __syncthreads();
+    if (c != 0 || lane >= task.ni) return;
+    const int i = task.i0 + lane;
     float4 g = task.force[i];
-    g.x = s[0] + e[0];
-    g.y = s[1] + e[1];
-    g.z = s[2] + e[2];
+    g.x = sum[0][lane];
+    g.y = sum[1][lane];
+    g.z = sum[2][lane];
     task.force[i] = g;
     float4 p = task.pm[i];
     float4 v = task.vm[i];
@@ -198,14 +204,14 @@ void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float
 {
     int nb, npad;
     batch_layout(n, &nb, &npad);
-    for (size_t i = 0; i < n; i += kWave) {
+    for (size_t i = 0; i < n; i += kBodyTaskBodies) {
         BodyTask t{};
         t.pm = pm;
         t.vm = vm;
         t.force = force;
         t.part = part;
         t.i0 = (int)i;
-        t.ni = (int)std::min<size_t>(kWave, n - i);
+        t.ni = (int)std::min<size_t>(kBodyTaskBodies, n - i);
         t.nb = nb;
         t.npad = npad;
         out.push_back(t);
@@ -243,7 +249,7 @@ hipError_t launch_reduce_integrate_batch(const BodyTask* tasks, int ntasks, floa
                                          unsigned* counter, hipStream_t stream)
 {
     if (ntasks <= 0) return hipSuccess;
-    k_reduce_integrate_batch<<<ntasks, kWave, 0, stream>>>(tasks, dt, damping, counter);
+    k_reduce_integrate_batch<<<ntasks, 3 * kBodyTaskBodies, 0, stream>>>(tasks, dt, damping, counter);
     return hipGetLastError();
 }
 

@@ -113,7 +113,7 @@ struct ForceTask {
     int reserved;
 };
 
-// Reduction + integration of up to 64 consecutive bodies of one system.
+// Reduction + integration of up to 256 consecutive bodies of one system.
 struct BodyTask {
     float4* pm;
     float4* vm;

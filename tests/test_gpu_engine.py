@@ -188,9 +188,14 @@ def test_ensemble_equals_symmetric_plan():
     """Batched systems of up to 2^18 bodies run the symmetric plan with 8
     bodies per lane in blocks of 512 (block pairs, diagonal pieces, slot sum
     in slot order): two batched steps equal two steps of a System with that
-    plan, bit for bit, for ragged sizes, a single block and a single body."""
+    plan, bit for bit, for ragged sizes, a single block and a single body,
+    with equal and with unequal masses."""
     sizes = [1, 63, 512, 513, 3000, 20000]
     systems = [ics.uniform_ball(n, seed=40 + k) for k, n in enumerate(sizes)]
+    rng = np.random.default_rng(41)
+    for pm, vm in systems[3:]:  # unequal masses U(0.5, 1), as the reference's own ICs
+        pm[:, 3] = rng.uniform(0.5, 1.0, pm.shape[0]).astype(np.float32)
+        vm[:, 3] = 1.0 / pm[:, 3]
     e = Ensemble(systems, EPS2)
     try:
         e.run(2, _native.BATCHED_GRAPH)
