@@ -90,6 +90,11 @@ def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s):
     med = float(np.median(times))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     algo = "CPU_SOA_tiled" if n % 4096 == 0 else "CPU_SOA"
+    # the reference's other CPU rows on the same positions, three calls each
+    others = {}
+    for name, fn in (("CPU_SOA", impl.cpu_soa), ("AVX", impl.cpu_avx), ("CPU_AOS", impl.cpu_aos)):
+        t = sorted(fn(pm, eps2)[1] for _ in range(3))
+        others[name] = round(n * n / (t[1] / 1e3) / 1e9, 3)
     return {
         "value": round(n * n / med / 1e9, 3),
         "unit": "10^9 interactions/s",
@@ -100,6 +105,7 @@ def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s):
                   f"{os.environ.get('OMP_PROC_BIND', '')}",
         "ms_per_force": round(med * 1e3, 2),
         "cpu": cpu_model(),
+        "other_cpu_rows": others,
     }
 
 
