@@ -57,6 +57,34 @@ def test_results_alone_equal_in_mix():
         assert np.array_equal(bits(vm), bits(mixed[k][1])), k
 
 
+def test_large_request_alone_equals_in_mix():
+    """A request above 2^18 bodies (batched as compensated one-sided rows)
+    next to small block-pair requests: its final state is bit-identical to
+    serving it alone, and so are the small ones'."""
+    big = _native.Request(0, 2 ** 18 + 300, 2, 11, _native.IC_PLUMMER, 1e-4, 0.0)
+    reqs = (_native.Request * 3)()
+    reqs[0] = big
+    reqs[1] = _native.Request(1, 700, 3, 12, _native.IC_UNIFORM_BALL, 0.01, 0.0)
+    reqs[2] = _native.Request(2, 5000, 2, 13, _native.IC_UNIFORM_BALL, 0.01, 0.0)
+    srv = serve.Server(max_active=4, keep_states=True)
+    try:
+        res, _, _ = srv.run(reqs)
+        assert all(r.status == 0 for r in res)
+        mixed = {k: srv.final_state(k, reqs[k].N) for k in range(3)}
+    finally:
+        srv.close()
+    for k in range(3):
+        alone = serve.Server(max_active=1, keep_states=True)
+        try:
+            res, _, _ = alone.run(one(reqs[k]))
+            assert res[0].status == 0
+            pm, vm = alone.final_state(k, reqs[k].N)
+        finally:
+            alone.close()
+        assert np.array_equal(bits(pm), bits(mixed[k][0])), k
+        assert np.array_equal(bits(vm), bits(mixed[k][1])), k
+
+
 def test_served_equals_ensemble_batched():
     """A served uniform-ball request against the same bodies stepped by the
     ensemble engine's batched path."""
