@@ -220,8 +220,57 @@ void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float
 
 void order_force_tasks(std::vector<ForceTask>& tasks)
 {
-    std::stable_sort(tasks.begin(), tasks.end(),
-                     [](const ForceTask& a, const ForceTask& b) { return task_cost(a) > task_cost(b); });
+    // The costs fall in three bands: a ROW_COMP task (len > 2^18) outweighs a
+    // PAIR task, which outweighs any ROW piece (len <= 512). A stable sort
+    // within the two row bands and the pairs as they come give the same order
+    // as a full stable sort by cost, in linear time for the many pairs.
+    std::vector<ForceTask> big, pairs, rows;
+    for (const ForceTask& t : tasks) {
+        if (t.kind == TASK_PAIR) pairs.push_back(t);
+        else if (t.kind == TASK_ROW_COMP) big.push_back(t);
+        else rows.push_back(t);
+    }
+    const auto by_cost = [](const ForceTask& a, const ForceTask& b) { return task_cost(a) > task_cost(b); };
+    std::stable_sort(big.begin(), big.end(), by_cost);
+    std::stable_sort(rows.begin(), rows.end(), by_cost);
+    tasks.clear();
+    tasks.insert(tasks.end(), big.begin(), big.end());
+    tasks.insert(tasks.end(), pairs.begin(), pairs.end());
+    tasks.insert(tasks.end(), rows.begin(), rows.end());
+}
+
+void build_force_task_bands(ForceTaskBands& out, const float4* pm, float* part, size_t n, float eps2)
+{
+    std::vector<ForceTask> all;
+    append_force_tasks(all, pm, part, n, eps2);
+    out.big.clear();
+    out.pairs.clear();
+    out.rows.clear();
+    for (const ForceTask& t : all) {
+        if (t.kind == TASK_PAIR) out.pairs.push_back(t);
+        else if (t.kind == TASK_ROW_COMP) out.big.push_back(t);
+        else out.rows.push_back(t);
+    }
+    // within one system the pieces of ragged blocks (shorter j ranges) go last
+    std::stable_sort(out.rows.begin(), out.rows.end(),
+                     [](const ForceTask& a, const ForceTask& b) { return a.len > b.len; });
+}
+
+void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<const ForceTaskBands*>& systems)
+{
+    out.clear();
+    size_t total = 0;
+    for (const ForceTaskBands* b : systems) total += b->big.size() + b->pairs.size() + b->rows.size();
+    out.reserve(total);
+    // the large systems' rows, largest system first
+    std::vector<const ForceTaskBands*> by_size;
+    for (const ForceTaskBands* b : systems)
+        if (!b->big.empty()) by_size.push_back(b);
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [](const ForceTaskBands* a, const ForceTaskBands* b) { return a->big[0].len > b->big[0].len; });
+    for (const ForceTaskBands* b : by_size) out.insert(out.end(), b->big.begin(), b->big.end());
+    for (const ForceTaskBands* b : systems) out.insert(out.end(), b->pairs.begin(), b->pairs.end());
+    for (const ForceTaskBands* b : systems) out.insert(out.end(), b->rows.begin(), b->rows.end());
 }
 
 int batch_force_blocks_per_cu()

@@ -439,6 +439,10 @@ struct ServeReq {
     int done = 0;
     int status = 0;
     float4* buf = nullptr;  // [posMass | velInvMass | force], 3 N records, then the slot planes
+    bool reserved = false;  // its bytes count against the budget
+    std::vector<float> hpm, hvm;  // initial conditions built on the host, until uploaded
+    ForceTaskBands tasks;         // its force tasks, built once it is on the device
+    std::vector<BodyTask> body_tasks;
     size_t bytes() const { return 3 * r.N * sizeof(float4) + batch_part_floats(r.N) * sizeof(float); }
     float* part() const { return reinterpret_cast<float*>(buf + 3 * r.N); }
 };
@@ -535,10 +539,10 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     const auto t0 = std::chrono::steady_clock::now();
     auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
     std::deque<ServeReq*> waiting;
+    std::vector<ServeReq*> staged;  // admitted, initial conditions built on the host, not yet on the device
     std::vector<ServeReq*> active;
     std::vector<ForceTask> host_ft;
     std::vector<BodyTask> host_bt;
-    std::vector<float> pm, vm;
     size_t next = 0, finished = 0, used = 0;
     double work = 0.0;
     bool dirty = true;
@@ -548,15 +552,22 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
         if (q->buf) {
             (void)hipFree(q->buf);
             q->buf = nullptr;
+        }
+        if (q->reserved) {
             used -= q->bytes();
+            q->reserved = false;
         }
         ++finished;
     };
-    while (finished < count && rc == OK) {
-        double t = now();
+    auto arrivals = [&] {
+        const double t = now();
         while (next < count && rq[order[next]].r.arrival_s <= t) waiting.push_back(&rq[order[next++]]);
-        // admit in arrival order while a slot is free and the budget holds
-        while (!waiting.empty() && (int)active.size() < s->max_active) {
+    };
+    // Admit in arrival order while a slot is free and the budget holds: the
+    // slot and the device bytes are reserved and the initial conditions are
+    // built on the host. This host work runs while a step is on the GPU.
+    auto admit = [&] {
+        while (!waiting.empty() && (int)(active.size() + staged.size()) < s->max_active) {
             ServeReq* q = waiting.front();
             if (q->bytes() > s->budget) {
                 q->status = 1;
@@ -565,34 +576,47 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
                 continue;
             }
             if (used + q->bytes() > s->budget) break;
-            if (request_ics(q->r, pm, vm)) {
+            waiting.pop_front();
+            if (request_ics(q->r, q->hpm, q->hvm)) {
                 q->status = 2;
-                waiting.pop_front();
                 retire(q, now());
                 continue;
             }
-            hipError_t e = hipMalloc(reinterpret_cast<void**>(&q->buf), q->bytes());
-            if (e == hipSuccess) e = hipMemcpy(q->buf, pm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
-            if (e == hipSuccess)
-                e = hipMemcpy(q->buf + q->r.N, vm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
-            if (e != hipSuccess) {
-                set_error("admitting request %llu failed: %s", (unsigned long long)q->r.id, hipGetErrorString(e));
-                if (q->buf) (void)hipFree(q->buf);
-                q->buf = nullptr;
-                rc = E_HIP;
-                break;
-            }
-            used += q->bytes();
             q->admit = now();
-            waiting.pop_front();
             if (q->r.steps == 0) {
                 retire(q, q->admit);
                 continue;
             }
+            used += q->bytes();
+            q->reserved = true;
+            staged.push_back(q);
+        }
+    };
+    while (finished < count && rc == OK) {
+        arrivals();
+        admit();
+        // staged requests onto the device while it is idle between steps
+        for (ServeReq* q : staged) {
+            hipError_t e = hipMalloc(reinterpret_cast<void**>(&q->buf), q->bytes());
+            if (e == hipSuccess)
+                e = hipMemcpy(q->buf, q->hpm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(q->buf + q->r.N, q->hvm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                set_error("admitting request %llu failed: %s", (unsigned long long)q->r.id, hipGetErrorString(e));
+                rc = E_HIP;
+                break;
+            }
+            std::vector<float>().swap(q->hpm);
+            std::vector<float>().swap(q->hvm);
+            build_force_task_bands(q->tasks, q->buf, q->part(), q->r.N, q->r.softeningSquared);
+            q->body_tasks.clear();
+            append_body_tasks(q->body_tasks, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->part(), q->r.N);
             active.push_back(q);
             dirty = true;
         }
         if (rc) break;
+        staged.clear();
         s->peak_active = std::max(s->peak_active, (int)active.size());
         if (active.empty()) {
             if (next < count) {
@@ -602,13 +626,15 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             continue;
         }
         if (dirty) {
-            host_ft.clear();
+            // the resident requests' prebuilt task lists, concatenated band by band
+            std::vector<const ForceTaskBands*> bands;
+            bands.reserve(active.size());
             host_bt.clear();
             for (ServeReq* q : active) {
-                append_force_tasks(host_ft, q->buf, q->part(), q->r.N, q->r.softeningSquared);
-                append_body_tasks(host_bt, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->part(), q->r.N);
+                bands.push_back(&q->tasks);
+                host_bt.insert(host_bt.end(), q->body_tasks.begin(), q->body_tasks.end());
             }
-            order_force_tasks(host_ft);
+            concat_force_task_bands(host_ft, bands);
             if (int r = upload(&s->ftasks, &s->ft_cap, host_ft); r) {
                 rc = r;
                 break;
@@ -619,18 +645,23 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             }
             dirty = false;
         }
-        // one step of every active request, then wait for it
+        // one step of every active request; arrivals and admissions (host
+        // initial conditions) overlap it; then wait for it
         hipError_t e = launch_force_batch(s->ftasks, (int)host_ft.size(), s->counter, s->blocks, s->stream);
         if (e == hipSuccess)
             e = launch_reduce_integrate_batch(s->btasks, (int)host_bt.size(), s->dt, s->damping, s->counter,
                                               s->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e == hipSuccess) {
+            arrivals();
+            admit();
+            e = hipStreamSynchronize(s->stream);
+        }
         if (e != hipSuccess) {
             set_error("server step: %s", hipGetErrorString(e));
             rc = E_HIP;
             break;
         }
-        t = now();
+        const double t = now();
         std::vector<ServeReq*> still;
         still.reserve(active.size());
         for (ServeReq* q : active) {
@@ -651,11 +682,17 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
                     rc = E_HIP;
                 }
             }
+            std::vector<ForceTask>().swap(q->tasks.big);
+            std::vector<ForceTask>().swap(q->tasks.pairs);
+            std::vector<ForceTask>().swap(q->tasks.rows);
+            std::vector<BodyTask>().swap(q->body_tasks);
             retire(q, t);
             dirty = true;
         }
         active.swap(still);
     }
+    for (ServeReq* q : staged)
+        if (q->buf) (void)hipFree(q->buf);
     for (ServeReq* q : active)
         if (q->buf) (void)hipFree(q->buf);
     for (size_t k = 0; k < count; ++k) {

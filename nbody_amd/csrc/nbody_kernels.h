@@ -136,6 +136,17 @@ void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float
 // Most expensive tasks first (stable), so the end of the launch holds the
 // cheapest ones.
 void order_force_tasks(std::vector<ForceTask>& tasks);
+
+// One system's force tasks split by cost band: compensated rows of a large
+// system (each outweighs a pair), block pairs, diagonal pieces (each lighter
+// than a pair). Concatenating the bands of many systems band by band gives a
+// most-expensive-first order without sorting, which is what a server that
+// rebuilds its task list whenever a request joins or leaves needs.
+struct ForceTaskBands {
+    std::vector<ForceTask> big, pairs, rows;
+};
+void build_force_task_bands(ForceTaskBands& out, const float4* pm, float* part, size_t n, float eps2);
+void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<const ForceTaskBands*>& systems);
 // Resident one-wave force workgroups per CU (the persistent grid is this
 // times the CU count).
 int batch_force_blocks_per_cu();
