@@ -145,9 +145,14 @@ __device__ __forceinline__ void dpp_steps(const float4& b, float eps2, const Lan
 {
     if constexpr (K < 16) {
 #pragma unroll
-        for (int u = 0; u < I; ++u)
-            body_body<NEWTON>(row_bcast<K>(b.x), row_bcast<K>(b.y), row_bcast<K>(b.z), row_bcast<K>(b.w),
-                              me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u], acc.z[u]);
+        for (int u = 0; u < I; ++u) {
+            if constexpr (NEWTON)
+                body_body<true>(row_bcast<K>(b.x), row_bcast<K>(b.y), row_bcast<K>(b.z), row_bcast<K>(b.w),
+                                me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u], acc.z[u]);
+            else
+                body_body_chain_dpp<K>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
+                                       acc.z[u]);
+        }
         dpp_steps<I, NEWTON, K + 1>(b, eps2, me, acc);
     }
 }
