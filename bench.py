@@ -446,7 +446,7 @@ def main():
     host_ms = sorted(hms[1:])[len(hms[1:]) // 2]
     out["host_entry"] = {"ms_per_call": round(host_ms, 4), "value": round(inter / (host_ms * 1e-3) / 1e9, 2),
                          "unit": "10^9 interactions/s",
-                         "what": "ComputeGravitation_HIP_Host on host buffers: H2D posMass + force + D2H, median of 5"}
+                         "what": "ComputeGravitation_HIP_Host on host buffers: H2D of posMass, force kernel, D2H of x, y, z; median of 5"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pm_h, vm_h, np.float32(eps2), dt, damping, args.cpu_budget_s)
     print(json.dumps(out))

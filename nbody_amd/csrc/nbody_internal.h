@@ -59,6 +59,31 @@ struct Buffer {
     }
 };
 
+// Grow-only page-locked host buffer (hipHostMalloc), the staging area of
+// the host-pointer entry: DMA runs at full link speed from and to it.
+struct PinnedBuffer {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) return e;
+            (void)hipHostFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+        }
+        hipError_t e = hipHostMalloc(&ptr, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = bytes;
+        return hipSuccess;
+    }
+};
+
 struct DeviceState {
     bool init = false;
     int cus = 0;
@@ -68,6 +93,7 @@ struct DeviceState {
     Buffer partial;   // FP32 slice partials
     Buffer partial64; // FP64 slice partials / energy partials
     Buffer stage_pm, stage_force, small;
+    PinnedBuffer host_stage;  // posMass in, force out, for the host-pointer entry
     // tuned plans: (requested variant, N, n, requested compensation, newton) -> plan
     std::map<std::tuple<int, size_t, size_t, int, int>, ForcePlan> tuned;
     // kernel timer: event pairs recorded around each force kernel while enabled

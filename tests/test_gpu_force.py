@@ -180,13 +180,17 @@ def test_mass_scaling_is_exact():
     assert np.array_equal(b[:, :3], 2.0 * a[:, :3])
 
 
-def test_host_twin_matches_device_entry():
-    pm, _ = bodies(3000, seed=8)
+@pytest.mark.parametrize("n", [1, 3000, 70000, 2000])
+def test_host_twin_matches_device_entry(n):
+    """The host-pointer entry (page-locked staging, growing and shrinking
+    across calls) gives the device entry's forces bit for bit and leaves the
+    caller's w untouched."""
+    pm, _ = bodies(n, seed=8)
     d_pm = dev(pm)
     f_dev, _ = run_entry("ComputeGravitation_HIP", d_pm, EPS2)
-    fh = np.full((3000, 4), np.nan, np.float32)
+    fh = np.full((n, 4), np.nan, np.float32)
     fh[:, 3] = -7.0
-    ms = _native.lib().ComputeGravitation_HIP_Host(fh.ctypes.data, np.ascontiguousarray(pm).ctypes.data, EPS2, 3000)
+    ms = _native.lib().ComputeGravitation_HIP_Host(fh.ctypes.data, np.ascontiguousarray(pm).ctypes.data, EPS2, n)
     assert ms > 0
     assert np.array_equal(fh.view(np.uint32), f_dev.view(np.uint32))
 
