@@ -109,6 +109,21 @@ def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s):
     }
 
 
+def crosscheck_cpu_aos(pm_host, force, f64, eps2):
+    """The reference's crosscheck (src/nbody.cu:313-325,406-418) of the GPU
+    force against CPU_AOS (its own build, or the oracle's restatement) on the
+    same positions: its componentwise relError maximum, and the per-body
+    normwise difference scaled by |F64|."""
+    sys.path.insert(0, os.path.join(HERE, "tests"))
+    import numpy as np
+    from oracle_lib import oracle, ref_cpu
+    impl = ref_cpu() or oracle()
+    aos, _ = impl.cpu_aos(np.ascontiguousarray(pm_host, np.float32), eps2)
+    d = np.linalg.norm(force[:, :3].astype(np.float64) - aos[:, :3], axis=1) / np.linalg.norm(f64[:, :3], axis=1)
+    return {"rel_error_max": float(oracle().max_rel_error(force, aos)), "normwise_max": float(d.max()),
+            "golden": "reference CPU_AOS" if ref_cpu() is not None else "oracle CPU_AOS"}
+
+
 KERNEL_NAMES = {0: "k_force_aos", 1: "k_force_lds", 2: "k_force_dpp", 3: "k_force_sgpr", 4: "k_force_sym"}
 
 
@@ -449,6 +464,8 @@ def main():
                          "what": "ComputeGravitation_HIP_Host on host buffers: H2D of posMass, force kernel, D2H of x, y, z; median of 5"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pm_h, vm_h, np.float32(eps2), dt, damping, args.cpu_budget_s)
+        out["accuracy"]["vs_cpu_aos"] = crosscheck_cpu_aos(pm.cpu().numpy(), step_force.cpu().numpy(),
+                                                           f64.cpu().numpy(), np.float32(eps2))
     print(json.dumps(out))
 
 
