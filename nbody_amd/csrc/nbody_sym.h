@@ -9,7 +9,8 @@
 // (DPP wave_ror:1), so in 64 steps each lane meets all 64 and the
 // accumulators come back home. Per pair: 16 VALU + 1 v_rsq_f32 for both
 // directions (the reaction on j carries m_i, where the reference's symmetric
-// kernels use m_j: src/nbody_GPU_AOS_tiled.cu:133-141). Waves visit the tiles
+// kernels use m_j: src/nbody_GPU_AOS_tiled.cu:133-141), or 14 VALU + 1
+// v_rsq_f32 when both blocks hold bodies of one mass. Waves visit the tiles
 // staggered (wave w takes tile (phase + w*A) mod T), so within a phase no two
 // waves own a tile and the per-tile reaction sums enter LDS in a fixed order,
 // with one barrier per phase: no atomics anywhere.
@@ -107,36 +108,71 @@ __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float
                    [n4] "v"(nm[4]), [n5] "v"(nm[5]), [n6] "v"(nm[6]), [n7] "v"(nm[7]), [eps] "s"(eps2));
 }
 
+// The same step for a block pair whose bodies all have one mass m0: the pair
+// weights are then +inv^3 (i side) and -inv^3 (j side) times m0, so the
+// chain drops the two mass products (14 VALU + 1 v_rsq_f32 per pair) and the
+// mass no longer rotates; sym_pair multiplies the partials by m0 when it
+// stores them. The reaction FMA negates d through the source modifier.
+// Bit-identical to the generic loop with EQ.
+#define NBODY_SYM_PAIR_EQ(u)                                                                           \
+    "v_sub_f32 %[dx], %[xj], %[x" #u "]\n"                                                             \
+    "v_sub_f32 %[dy], %[yj], %[y" #u "]\n"                                                             \
+    "v_sub_f32 %[dz], %[zj], %[z" #u "]\n"                                                             \
+    "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                                           \
+    "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                                  \
+    "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                                  \
+    "v_rsq_f32 %[s], %[s]\n"                                                                           \
+    "s_nop 0\n"                                                                                        \
+    "v_mul_f32 %[t], %[s], %[s]\n"                                                                     \
+    "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
+    "v_fma_f32 %[rx], -%[dx], %[t], %[rx]\n"                                                           \
+    "v_fma_f32 %[ry], -%[dy], %[t], %[ry]\n"                                                           \
+    "v_fma_f32 %[rz], -%[dz], %[t], %[rz]\n"                                                           \
+    "v_fmac_f32 %[ax" #u "], %[dx], %[t]\n"                                                            \
+    "v_fmac_f32 %[ay" #u "], %[dy], %[t]\n"                                                            \
+    "v_fmac_f32 %[az" #u "], %[dz], %[t]\n"
+
+__device__ __forceinline__ void sym_step8_eq(float& xj, float& yj, float& zj, float& rx, float& ry, float& rz,
+                                             Lanes<8>& acc, const Lanes<8>& me, float eps2)
+{
+    float dx, dy, dz, s, t;
+    asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3)
+                 NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ(7)
+                 NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj)
+                 NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+                 : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [rx] "+v"(rx), [ry] "+v"(ry),
+                   [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                   [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
+                   [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
+                   [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
+                   [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
+                   [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
+                   [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s),
+                   [t] "=&v"(t)
+                 : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
+                   [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
+                   [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
+                   [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
+                   [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
+                   [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]), [eps] "s"(eps2));
+}
+
+#undef NBODY_SYM_PAIR_EQ
 #undef NBODY_SYM_PAIR
 #undef NBODY_ROR1
 
-// Block pair (P, Q), P < Q. `jacc` is the workgroup's LDS reaction buffer.
-// P is never the last (possibly ragged) block, so every i lane holds a real
-// body; j bodies past n enter with zero mass.
-template <int A, int W, bool NEWTON>
-__device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, int npad, float eps2,
-                                         float* __restrict__ part, int P, int Q, float (*jacc)[64 * A * W])
+// The phases of block pair (P, Q): every wave streams the T tiles of block Q
+// past its A i bodies per lane, adding the i side into acc and each tile's
+// reaction into jacc. With EQ the pair weights leave out the (common) mass.
+template <int A, int W, bool NEWTON, bool EQ>
+__device__ __forceinline__ void sym_phases(const float4* __restrict__ pm, int n, float eps2, int Q,
+                                           const Lanes<A>& me, const float (&nm)[A], Lanes<A>& acc,
+                                           float (*jacc)[64 * A * W])
 {
     constexpr int B = 64 * A * W;
     constexpr int T = A * W;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const size_t plane = (size_t)(npad / B) * npad;
-
-    Lanes<A> me, acc;
-    float nm[A];
-#pragma unroll
-    for (int u = 0; u < A; ++u) {
-        const float4 q = pm[P * B + u * 64 * W + threadIdx.x];
-        me.x[u] = q.x;
-        me.y[u] = q.y;
-        me.z[u] = q.z;
-        nm[u] = -q.w;
-    }
-    zero<A>(acc);
-
-    for (int k = threadIdx.x; k < B; k += 64 * W) jacc[0][k] = jacc[1][k] = jacc[2][k] = 0.f;
-    __syncthreads();
     for (int ph = 0; ph < T; ++ph) {
         const int tile = (ph + wave * A) % T;
         const int jbase = Q * B + tile * 64;  // wave-uniform
@@ -151,7 +187,8 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
             float rx = 0.f, ry = 0.f, rz = 0.f;
             for (int k = 0; k < 64; ++k) {
                 if constexpr (A == 8 && !NEWTON) {
-                    sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
+                    if constexpr (EQ) sym_step8_eq(xj, yj, zj, rx, ry, rz, acc, me, eps2);
+                    else sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
                     continue;
                 }
 #pragma unroll
@@ -164,8 +201,8 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
                     d2 = __builtin_fmaf(dz, dz, d2);
                     const float inv = rsqrt_f32<NEWTON>(d2);
                     const float inv3 = (inv * inv) * inv;
-                    const float si = mj * inv3;
-                    const float sj = nm[u] * inv3;
+                    const float si = EQ ? inv3 : mj * inv3;
+                    const float sj = EQ ? -inv3 : nm[u] * inv3;
                     acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
                     acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
                     acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
@@ -176,7 +213,7 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
                 xj = wave_ror1(xj);
                 yj = wave_ror1(yj);
                 zj = wave_ror1(zj);
-                mj = wave_ror1(mj);
+                if constexpr (!EQ) mj = wave_ror1(mj);
                 rx = wave_ror1(rx);
                 ry = wave_ror1(ry);
                 rz = wave_ror1(rz);
@@ -188,18 +225,69 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
         }
         __syncthreads();
     }
+}
+
+// True on every thread when `ok` holds on every thread of the workgroup.
+template <int W>
+__device__ __forceinline__ bool workgroup_all(bool ok)
+{
+    if constexpr (W == 1) return __builtin_amdgcn_ballot_w64(!ok) == 0;
+    else return __builtin_amdgcn_readfirstlane(__syncthreads_and(ok)) != 0;
+}
+
+// Block pair (P, Q), P < Q. `jacc` is the workgroup's LDS reaction buffer.
+// P is never the last (possibly ragged) block, so every i lane holds a real
+// body; j bodies past n enter with zero mass. When every body of both blocks
+// has the mass of body P*B (uniform ICs: unit masses, m = 1/N), the pair
+// weights leave the mass out and the partials are multiplied by it once, at
+// the store; with m = 1 (or any power of two) the result is bit-identical to
+// the general weights.
+template <int A, int W, bool NEWTON>
+__device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, int npad, float eps2,
+                                         float* __restrict__ part, int P, int Q, float (*jacc)[64 * A * W])
+{
+    constexpr int B = 64 * A * W;
+    const size_t plane = (size_t)(npad / B) * npad;
+    const float m0 = pm[P * B].w;
+
+    Lanes<A> me, acc;
+    float nm[A];
+    bool same = true;
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+        const float4 q = pm[P * B + u * 64 * W + threadIdx.x];
+        me.x[u] = q.x;
+        me.y[u] = q.y;
+        me.z[u] = q.z;
+        nm[u] = -q.w;
+        same &= q.w == m0;
+    }
+    // all A loads in flight at once (no short-circuit), clamped into the array
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+        const int j = Q * B + u * 64 * W + (int)threadIdx.x;
+        same &= (j < n) & (pm[min(j, n - 1)].w == m0);
+    }
+    zero<A>(acc);
+
+    for (int k = threadIdx.x; k < B; k += 64 * W) jacc[0][k] = jacc[1][k] = jacc[2][k] = 0.f;
+    __syncthreads();
+    const bool eq = workgroup_all<W>(same);
+    if (eq) sym_phases<A, W, NEWTON, true>(pm, n, eps2, Q, me, nm, acc, jacc);
+    else sym_phases<A, W, NEWTON, false>(pm, n, eps2, Q, me, nm, acc, jacc);
+    const float scale = eq ? m0 : 1.0f;
     for (int k = threadIdx.x; k < B; k += 64 * W) {
         const size_t o = (size_t)P * npad + (size_t)Q * B + k;
-        part[o] = jacc[0][k];
-        part[plane + o] = jacc[1][k];
-        part[2 * plane + o] = jacc[2][k];
+        part[o] = jacc[0][k] * scale;
+        part[plane + o] = jacc[1][k] * scale;
+        part[2 * plane + o] = jacc[2][k] * scale;
     }
 #pragma unroll
     for (int u = 0; u < A; ++u) {
         const size_t o = (size_t)Q * npad + (size_t)P * B + u * 64 * W + threadIdx.x;
-        part[o] = acc.x[u];
-        part[plane + o] = acc.y[u];
-        part[2 * plane + o] = acc.z[u];
+        part[o] = acc.x[u] * scale;
+        part[plane + o] = acc.y[u] * scale;
+        part[2 * plane + o] = acc.z[u] * scale;
     }
 }
 

@@ -115,6 +115,34 @@ def test_symmetric_shapes(oracle, shape, newton):
         assert np.all(f[:, 3] == -7.0)
 
 
+@pytest.mark.parametrize("shape", [(8, 512), (4, 256), (2, 512)])
+def test_symmetric_equal_mass_blocks(oracle, shape):
+    """Block pairs whose bodies all share one mass take the mass-free pair
+    weights and scale the partials once. Cases: one mass everywhere (not a
+    power of two), the same with one heavier body in block 2 (pairs with block
+    2 take the general weights, the rest the equal-mass ones), a ragged N
+    (the last block holds zero-mass padding), and power-of-two scaling of the
+    common mass, which must scale the forces exactly."""
+    a, b = shape
+    for n in (4 * b, 4 * b + 37):
+        pm, _ = bodies(n, seed=9)
+        pm[:, 3] = 0.37
+        mixed = pm.copy()
+        mixed[2 * b + 5, 3] = 0.6
+        for case, x in (("one mass", pm), ("one heavier body", mixed)):
+            f = run_force(dev(x), EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b)
+            mx, rms = force_errors(f, oracle.fp64_forces(x, EPS2))
+            assert mx <= TOL_MAX and rms <= TOL_RMS, (shape, n, case, mx, rms)
+            assert np.all(f[:, 3] == -7.0)
+        unit = pm.copy()
+        unit[:, 3] = 1.0
+        quarter = pm.copy()
+        quarter[:, 3] = 0.25
+        f1 = run_force(dev(unit), EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b)
+        fq = run_force(dev(quarter), EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b)
+        assert np.array_equal(fq[:, :3], 0.25 * f1[:, :3]), (shape, n)
+
+
 def test_symmetric_rejects_i_ranges():
     pm, _ = bodies(3000, seed=3)
     d = dev(pm)
