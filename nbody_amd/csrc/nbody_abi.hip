@@ -120,8 +120,8 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
 }
 
 // The untuned choice for NBODY_HIP_AUTO: the symmetric kernel for whole
-// systems of at least two of its blocks (half the pairs, 16 VALU + 1 rsq per
-// pair); otherwise the LDS broadcast, whose VALU operands all sit in VGPRs
+// systems of at least two of its blocks (half the pairs, 14-16 VALU + 1 rsq
+// per pair); otherwise the LDS broadcast, whose VALU operands all sit in VGPRs
 // (gfx950 issues those faster than ops reading an SGPR).
 int auto_variant(size_t N, size_t n)
 {
