@@ -146,8 +146,8 @@ int validate_plan(const ForcePlan& p, size_t N)
 {
     if (p.variant == VAR_SYMMETRIC) {
         if (!symmetric_shape_ok(p.ipl, p.jslice)) {
-            set_error("symmetric plan needs i_per_lane 2/4/8 and a block of 64*i_per_lane*{1,2,4} bodies "
-                      "(got %d, %d)", p.ipl, p.jslice);
+            set_error("symmetric plan needs i_per_lane 2 with blocks of 128 or 512 bodies, or 4 or 8 with "
+                      "blocks of 64*i_per_lane*{1,2,4} (got %d, %d)", p.ipl, p.jslice);
             return E_ARG;
         }
         if (p.occ < 0 || p.occ > 8) {
@@ -510,7 +510,7 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
         if (fam == VAR_AOS) {
             cands.push_back(default_plan(fam, N, N, d->cus, &o));
         } else if (fam == VAR_SYMMETRIC) {
-            const int shapes[][2] = {{2, 512}, {4, 256}, {4, 512}, {4, 1024}, {8, 512}, {8, 1024}, {8, 2048}};
+            const int shapes[][2] = {{2, 128}, {2, 512}, {4, 256}, {4, 512}, {4, 1024}, {8, 512}, {8, 1024}, {8, 2048}};
             for (const auto& sh : shapes) {
                 o.i_per_lane = sh[0];
                 o.jslice = sh[1];

@@ -506,14 +506,15 @@ bool symmetric_shape_ok(int ipl, int block)
 {
     const int w = block / (64 * ipl);
     if (w * 64 * ipl != block) return false;
-    return (ipl == 2 && w == 4) || (ipl == 4 && (w == 1 || w == 2 || w == 4)) || (ipl == 8 && (w == 1 || w == 2 || w == 4));
+    return (ipl == 2 && (w == 1 || w == 4)) || (ipl == 4 && (w == 1 || w == 2 || w == 4)) ||
+           (ipl == 8 && (w == 1 || w == 2 || w == 4));
 }
 
 int symmetric_natural_occupancy(const ForcePlan& plan)
 {
     if (plan.variant != VAR_SYMMETRIC || !symmetric_shape_ok(plan.ipl, plan.jslice)) return 0;
     const int w = plan.jslice / (64 * plan.ipl);
-    if (plan.ipl == 2) return natural_occupancy<2, 4>(plan.newton);
+    if (plan.ipl == 2) return w == 1 ? natural_occupancy<2, 1>(plan.newton) : natural_occupancy<2, 4>(plan.newton);
     if (plan.ipl == 4)
         return w == 1 ? natural_occupancy<4, 1>(plan.newton) : w == 2 ? natural_occupancy<4, 2>(plan.newton)
                                                                       : natural_occupancy<4, 4>(plan.newton);
@@ -559,7 +560,7 @@ hipError_t launch_force(const ForcePlan& plan, const ForceParams& p, hipStream_t
         sp.eps2 = p.eps2;
         sp.part = reinterpret_cast<float*>(p.hi);
         const int w = plan.jslice / (64 * plan.ipl);
-        if (plan.ipl == 2) e = go_sym<2, 4>(plan, sp, stream);
+        if (plan.ipl == 2) e = w == 1 ? go_sym<2, 1>(plan, sp, stream) : go_sym<2, 4>(plan, sp, stream);
         else if (plan.ipl == 4) e = w == 1 ? go_sym<4, 1>(plan, sp, stream) : w == 2 ? go_sym<4, 2>(plan, sp, stream)
                                                                                : go_sym<4, 4>(plan, sp, stream);
         else e = w == 1 ? go_sym<8, 1>(plan, sp, stream) : w == 2 ? go_sym<8, 2>(plan, sp, stream)

@@ -48,7 +48,7 @@ def entry_plan(entry, n):
 
 ENTRIES = ["ComputeGravitation_HIP_AOS", "ComputeGravitation_HIP_LDS", "ComputeGravitation_HIP_DPP",
            "ComputeGravitation_HIP_SGPR", "ComputeGravitation_HIP_Symmetric", "ComputeGravitation_HIP"]
-SYM_SHAPES = [(2, 512), (4, 256), (4, 512), (4, 1024), (8, 512), (8, 1024), (8, 2048)]
+SYM_SHAPES = [(2, 128), (2, 512), (4, 256), (4, 512), (4, 1024), (8, 512), (8, 1024), (8, 2048)]
 SIZES = [1, 2, 3, 17, 255, 256, 257, 1000, 1024, 4097, 16384]
 
 
@@ -115,7 +115,7 @@ def test_symmetric_shapes(oracle, shape, newton):
         assert np.all(f[:, 3] == -7.0)
 
 
-@pytest.mark.parametrize("shape", [(8, 512), (4, 256), (2, 512)])
+@pytest.mark.parametrize("shape", [(8, 512), (4, 256), (2, 512), (2, 128)])
 def test_symmetric_equal_mass_blocks(oracle, shape):
     """Block pairs whose bodies all share one mass take the mass-free pair
     weights and scale the partials once. Cases: one mass everywhere (not a
