@@ -57,10 +57,12 @@ enum {
 /* count systems of N[s] bodies, stored back to back in posMass/velInvMass
  * (system s starts at body sum_{r<s} N[r]); softeningSquared[s] per system;
  * dt and damping shared. The batched modes evaluate a system of N <= 2^18
- * bodies pairwise-symmetrically in blocks of 512 (each pair once, as
- * ComputeGravitation_HIP_Symmetric with 8 bodies per lane: bit-identical to
- * it), keeping 12 * ceil(N/512)^2 * 512 bytes of block partials in HBM; a
- * larger one one-sidedly with compensated sums (12 N bytes of partials). */
+ * bodies pairwise-symmetrically in blocks of 512, each pair once (block
+ * pairs exactly as ComputeGravitation_HIP_Symmetric with 8 bodies per lane;
+ * each diagonal block as one task, where that entry sums it one-sidedly, so
+ * the two agree to rounding), keeping 12 * ceil(N/512)^2 * 512 bytes of
+ * block partials in HBM; a larger one one-sidedly with compensated sums
+ * (12 N bytes of partials). */
 nbody_hip_ensemble *nbody_hip_ensemble_create(size_t count, const size_t *N, const float *posMass,
                                               const float *velInvMass, const float *softeningSquared, float dt,
                                               float damping);

@@ -1,6 +1,6 @@
 // Batched force and integrate kernels for many independent systems: a
 // persistent grid of one-wave workgroups pulls ForceTasks (block pairs,
-// diagonal pieces, compensated rows) from a device work queue, then one
+// diagonal blocks, compensated rows) from a device work queue, then one
 // launch adds every body's slots and integrates it. Task lists are built on
 // the host, most expensive first.
 #include "nbody_device.h"
@@ -35,29 +35,29 @@ __global__ __launch_bounds__(kWave) void k_force_batch(const ForceTask* __restri
             sym_pair<kPairA, 1, false>(task.pm, task.n, task.npad, task.eps2, task.part, task.a, task.b, jacc);
             continue;
         }
+        if (task.kind == TASK_DIAG) {
+            sym_diag_block<false>(task.pm, task.n, task.npad, task.eps2, task.part, task.a);
+            continue;
+        }
+        // TASK_ROW_COMP: 64 bodies against task.len bodies, TwoSum-folded chunks
         const int i = task.a + (int)threadIdx.x;
         const float4 q = task.pm[min(i, task.n - 1)];
-        Lanes<1> me, hi;
+        Lanes<1> me, hi, lo;
         me.x[0] = q.x;
         me.y[0] = q.y;
         me.z[0] = q.z;
         zero<1>(hi);
+        zero<1>(lo);
         const float4* src = task.pm + task.b;
-        if (task.kind == TASK_ROW) {
-            sweep_const<1, false>(src, task.len, task.eps2, me, hi);
-        } else {
-            Lanes<1> lo;
-            zero<1>(lo);
-            for (int c = 0; c < task.len; c += kChunk) {
-                Lanes<1> part;
-                zero<1>(part);
-                sweep_const<1, false>(src + c, min(kChunk, task.len - c), task.eps2, me, part);
-                fold<1>(hi, lo, part);
-            }
-            hi.x[0] += lo.x[0];
-            hi.y[0] += lo.y[0];
-            hi.z[0] += lo.z[0];
+        for (int c = 0; c < task.len; c += kChunk) {
+            Lanes<1> part;
+            zero<1>(part);
+            sweep_const<1, false>(src + c, min(kChunk, task.len - c), task.eps2, me, part);
+            fold<1>(hi, lo, part);
         }
+        hi.x[0] += lo.x[0];
+        hi.y[0] += lo.y[0];
+        hi.z[0] += lo.z[0];
         if (i < task.n) {
             const size_t plane = (size_t)task.nb * task.npad;
             const size_t o = (size_t)task.slot * task.npad + i;
@@ -134,6 +134,7 @@ __global__ __launch_bounds__(3 * kBodyTaskBodies) void k_reduce_integrate_batch(
 double task_cost(const ForceTask& t)
 {
     if (t.kind == TASK_PAIR) return (double)kBatchBlock * kBatchBlock / kWave * 17.0;
+    if (t.kind == TASK_DIAG) return (double)t.len * t.len / 2 / kWave * 17.0;
     return (double)t.len * 13.0;
 }
 
@@ -186,16 +187,13 @@ void append_force_tasks(std::vector<ForceTask>& out, const float4* pm, float* pa
             t.b = q;
             out.push_back(t);
         }
-    t.kind = TASK_ROW;
+    t.kind = TASK_DIAG;
+    t.b = 0;
     for (int p = 0; p < nb; ++p) {
-        const int j0 = p * kBatchBlock;
-        t.b = j0;
-        t.len = (int)std::min<size_t>(kBatchBlock, n - j0);
+        t.a = p;
+        t.len = (int)std::min<size_t>(kBatchBlock, n - (size_t)p * kBatchBlock);
         t.slot = p;
-        for (int i = j0; i < j0 + t.len; i += kWave) {
-            t.a = i;
-            out.push_back(t);
-        }
+        out.push_back(t);
     }
 }
 
@@ -221,22 +219,23 @@ void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float
 void order_force_tasks(std::vector<ForceTask>& tasks)
 {
     // The costs fall in three bands: a ROW_COMP task (len > 2^18) outweighs a
-    // PAIR task, which outweighs any ROW piece (len <= 512). A stable sort
-    // within the two row bands and the pairs as they come give the same order
-    // as a full stable sort by cost, in linear time for the many pairs.
-    std::vector<ForceTask> big, pairs, rows;
+    // PAIR task, which outweighs any DIAG task (at most 512 bodies, each pair
+    // once). A stable sort within the two outer bands and the pairs as they
+    // come give the same order as a full stable sort by cost, in linear time
+    // for the many pairs.
+    std::vector<ForceTask> big, pairs, diag;
     for (const ForceTask& t : tasks) {
         if (t.kind == TASK_PAIR) pairs.push_back(t);
         else if (t.kind == TASK_ROW_COMP) big.push_back(t);
-        else rows.push_back(t);
+        else diag.push_back(t);
     }
     const auto by_cost = [](const ForceTask& a, const ForceTask& b) { return task_cost(a) > task_cost(b); };
     std::stable_sort(big.begin(), big.end(), by_cost);
-    std::stable_sort(rows.begin(), rows.end(), by_cost);
+    std::stable_sort(diag.begin(), diag.end(), by_cost);
     tasks.clear();
     tasks.insert(tasks.end(), big.begin(), big.end());
     tasks.insert(tasks.end(), pairs.begin(), pairs.end());
-    tasks.insert(tasks.end(), rows.begin(), rows.end());
+    tasks.insert(tasks.end(), diag.begin(), diag.end());
 }
 
 void build_force_task_bands(ForceTaskBands& out, const float4* pm, float* part, size_t n, float eps2)
@@ -245,14 +244,14 @@ void build_force_task_bands(ForceTaskBands& out, const float4* pm, float* part, 
     append_force_tasks(all, pm, part, n, eps2);
     out.big.clear();
     out.pairs.clear();
-    out.rows.clear();
+    out.diag.clear();
     for (const ForceTask& t : all) {
         if (t.kind == TASK_PAIR) out.pairs.push_back(t);
         else if (t.kind == TASK_ROW_COMP) out.big.push_back(t);
-        else out.rows.push_back(t);
+        else out.diag.push_back(t);
     }
-    // within one system the pieces of ragged blocks (shorter j ranges) go last
-    std::stable_sort(out.rows.begin(), out.rows.end(),
+    // within one system a ragged last block (fewer bodies) goes last
+    std::stable_sort(out.diag.begin(), out.diag.end(),
                      [](const ForceTask& a, const ForceTask& b) { return a.len > b.len; });
 }
 
@@ -260,7 +259,7 @@ void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<cons
 {
     out.clear();
     size_t total = 0;
-    for (const ForceTaskBands* b : systems) total += b->big.size() + b->pairs.size() + b->rows.size();
+    for (const ForceTaskBands* b : systems) total += b->big.size() + b->pairs.size() + b->diag.size();
     out.reserve(total);
     // the large systems' rows, largest system first
     std::vector<const ForceTaskBands*> by_size;
@@ -270,7 +269,7 @@ void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<cons
                      [](const ForceTaskBands* a, const ForceTaskBands* b) { return a->big[0].len > b->big[0].len; });
     for (const ForceTaskBands* b : by_size) out.insert(out.end(), b->big.begin(), b->big.end());
     for (const ForceTaskBands* b : systems) out.insert(out.end(), b->pairs.begin(), b->pairs.end());
-    for (const ForceTaskBands* b : systems) out.insert(out.end(), b->rows.begin(), b->rows.end());
+    for (const ForceTaskBands* b : systems) out.insert(out.end(), b->diag.begin(), b->diag.end());
 }
 
 int batch_force_blocks_per_cu()

@@ -684,7 +684,7 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             }
             std::vector<ForceTask>().swap(q->tasks.big);
             std::vector<ForceTask>().swap(q->tasks.pairs);
-            std::vector<ForceTask>().swap(q->tasks.rows);
+            std::vector<ForceTask>().swap(q->tasks.diag);
             std::vector<BodyTask>().swap(q->body_tasks);
             retire(q, t);
             dirty = true;

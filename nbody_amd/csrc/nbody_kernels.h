@@ -85,8 +85,8 @@ namespace nbody {
 // workgroups take ForceTasks from a device work queue (an atomic counter).
 // A system of n <= kBatchPairMaxN bodies is cut into blocks of kBatchBlock
 // and runs as the symmetric plan does (nbody_sym.h): one PAIR task per block
-// pair P < Q and kBatchBlock/64 one-sided ROW pieces per diagonal block, each
-// writing its own slot of the system's partial planes part[3][nb][npad]. A
+// pair P < Q and one DIAG task per diagonal block, each writing its own slot
+// of the system's partial planes part[3][nb][npad]. A
 // larger system (its slot planes would grow as n^2/kBatchBlock) runs as
 // ROW_COMP tasks: 64 bodies against all n with TwoSum-folded chunks, into a
 // single slot. A second launch adds each body's slots in slot order (TwoSum)
@@ -96,7 +96,7 @@ namespace nbody {
 constexpr int kBatchBlock = 512;                     // 8 bodies per lane, one wave
 constexpr size_t kBatchPairMaxN = size_t(1) << 18;   // largest system run as block pairs
 
-enum BatchTaskKind : int { TASK_PAIR = 0, TASK_ROW = 1, TASK_ROW_COMP = 2 };
+enum BatchTaskKind : int { TASK_PAIR = 0, TASK_DIAG = 1, TASK_ROW_COMP = 2 };
 
 struct ForceTask {
     const float4* pm;  // the system's bodies
@@ -106,10 +106,10 @@ struct ForceTask {
     int npad;          // length of one slot row
     float eps2;
     int kind;          // BatchTaskKind
-    int a;             // PAIR: block P; ROW*: first i body (64 of them)
-    int b;             // PAIR: block Q; ROW*: first j body
-    int len;           // ROW*: number of j bodies
-    int slot;          // ROW*: slot written
+    int a;             // PAIR, DIAG: block P; ROW_COMP: first i body (64 of them)
+    int b;             // PAIR: block Q; ROW_COMP: first j body
+    int len;           // DIAG: bodies in the block; ROW_COMP: number of j bodies
+    int slot;          // slot written (DIAG: P; ROW_COMP: 0)
     int reserved;
 };
 
@@ -138,12 +138,12 @@ void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float
 void order_force_tasks(std::vector<ForceTask>& tasks);
 
 // One system's force tasks split by cost band: compensated rows of a large
-// system (each outweighs a pair), block pairs, diagonal pieces (each lighter
-// than a pair). Concatenating the bands of many systems band by band gives a
+// system (each outweighs a pair), block pairs, diagonal blocks (each about
+// half a pair). Concatenating the bands of many systems band by band gives a
 // most-expensive-first order without sorting, which is what a server that
 // rebuilds its task list whenever a request joins or leaves needs.
 struct ForceTaskBands {
-    std::vector<ForceTask> big, pairs, rows;
+    std::vector<ForceTask> big, pairs, diag;
 };
 void build_force_task_bands(ForceTaskBands& out, const float4* pm, float* part, size_t n, float eps2);
 void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<const ForceTaskBands*>& systems);

@@ -15,9 +15,10 @@
 // waves own a tile and the per-tile reaction sums enter LDS in a fixed order,
 // with one barrier per phase: no atomics anywhere.
 //
-// The diagonal of block P (its bodies against each other) is one-sided and
-// split into A pieces of 64*W bodies each, scheduled after every pair so that
-// these short tasks fill the end of the launch.
+// The diagonal of block P (its bodies against each other) comes after every
+// pair: in the single-system kernel as A one-sided pieces of 64*W bodies
+// each (sym_diag_piece), in the batched kernel as one task that takes each
+// pair of the block once (sym_diag_block, blocks of 512 on one wave).
 //
 // Every body then has exactly one partial per block slot: slot Q for the i
 // side of (P, Q), slot P for its j side, slot P for the diagonal. The partials
@@ -132,6 +133,50 @@ __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float
     "v_fmac_f32 %[ay" #u "], %[dy], %[t]\n"                                                            \
     "v_fmac_f32 %[az" #u "], %[dz], %[t]\n"
 
+// The pair chains with the weight s^3 cleared on the lanes whose bit in
+// %[mask] is 0 (v_cndmask right after the cube): those lanes add exact zeros.
+// Used for the one step of a diagonal tile against itself that would
+// otherwise take each pair twice.
+#define NBODY_SYM_PAIR_M(u)                                                                            \
+    "v_sub_f32 %[dx], %[xj], %[x" #u "]\n"                                                             \
+    "v_sub_f32 %[dy], %[yj], %[y" #u "]\n"                                                             \
+    "v_sub_f32 %[dz], %[zj], %[z" #u "]\n"                                                             \
+    "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                                           \
+    "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                                  \
+    "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                                  \
+    "v_rsq_f32 %[s], %[s]\n"                                                                           \
+    "s_nop 0\n"                                                                                        \
+    "v_mul_f32 %[t], %[s], %[s]\n"                                                                     \
+    "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
+    "v_cndmask_b32 %[t], 0, %[t], %[mask]\n"                                                           \
+    "v_mul_f32 %[s], %[mj], %[t]\n"                                                                    \
+    "v_mul_f32 %[t], %[n" #u "], %[t]\n"                                                               \
+    "v_fmac_f32 %[rx], %[dx], %[t]\n"                                                                  \
+    "v_fmac_f32 %[ry], %[dy], %[t]\n"                                                                  \
+    "v_fmac_f32 %[rz], %[dz], %[t]\n"                                                                  \
+    "v_fmac_f32 %[ax" #u "], %[dx], %[s]\n"                                                            \
+    "v_fmac_f32 %[ay" #u "], %[dy], %[s]\n"                                                            \
+    "v_fmac_f32 %[az" #u "], %[dz], %[s]\n"
+
+#define NBODY_SYM_PAIR_EQ_M(u)                                                                         \
+    "v_sub_f32 %[dx], %[xj], %[x" #u "]\n"                                                             \
+    "v_sub_f32 %[dy], %[yj], %[y" #u "]\n"                                                             \
+    "v_sub_f32 %[dz], %[zj], %[z" #u "]\n"                                                             \
+    "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                                           \
+    "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                                  \
+    "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                                  \
+    "v_rsq_f32 %[s], %[s]\n"                                                                           \
+    "s_nop 0\n"                                                                                        \
+    "v_mul_f32 %[t], %[s], %[s]\n"                                                                     \
+    "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
+    "v_cndmask_b32 %[t], 0, %[t], %[mask]\n"                                                           \
+    "v_fma_f32 %[rx], -%[dx], %[t], %[rx]\n"                                                           \
+    "v_fma_f32 %[ry], -%[dy], %[t], %[ry]\n"                                                           \
+    "v_fma_f32 %[rz], -%[dz], %[t], %[rz]\n"                                                           \
+    "v_fmac_f32 %[ax" #u "], %[dx], %[t]\n"                                                            \
+    "v_fmac_f32 %[ay" #u "], %[dy], %[t]\n"                                                            \
+    "v_fmac_f32 %[az" #u "], %[dz], %[t]\n"
+
 __device__ __forceinline__ void sym_step8_eq(float& xj, float& yj, float& zj, float& rx, float& ry, float& rz,
                                              Lanes<8>& acc, const Lanes<8>& me, float eps2)
 {
@@ -223,6 +268,90 @@ __device__ __forceinline__ void sym_step2_eq(float& xj, float& yj, float& zj, fl
                  : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [z0] "v"(me.z[0]),
                    [z1] "v"(me.z[1]), [eps] "s"(eps2));
 }
+
+// One rotation step of diagonal tile T (see sym_diag_phases) in assembly, in
+// the chain order of sym_step8: the pairs with the i sub-blocks u < T, then
+// (MODE 1) the tile's own sub-block, or (MODE 2) the same with the weights of
+// the lanes outside `mask` cleared; then the rotation. Every operand of
+// sym_step8 is bound (they are live through the task anyway), so the
+// registers stay those of the block-pair step. With EQ the mass neither
+// enters nor rotates.
+#define NBODY_DIAG_OPS \
+    : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry), [rz] "+v"(rz), \
+      [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]), [ax3] "+v"(acc.x[3]), \
+      [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]), [ax7] "+v"(acc.x[7]), \
+      [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]), [ay3] "+v"(acc.y[3]), \
+      [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]), [ay7] "+v"(acc.y[7]), \
+      [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]), [az3] "+v"(acc.z[3]), \
+      [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]), [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), \
+      [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s), [t] "=&v"(t) \
+    : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]), [x4] "v"(me.x[4]), \
+      [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]), [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), \
+      [y2] "v"(me.y[2]), [y3] "v"(me.y[3]), [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), \
+      [y7] "v"(me.y[7]), [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]), \
+      [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]), [n0] "v"(nm[0]), \
+      [n1] "v"(nm[1]), [n2] "v"(nm[2]), [n3] "v"(nm[3]), [n4] "v"(nm[4]), [n5] "v"(nm[5]), [n6] "v"(nm[6]), \
+      [n7] "v"(nm[7]), [eps] "s"(eps2), [mask] "s"(mask)
+
+template <int T, int MODE, bool EQ>
+__device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry,
+                                               float& rz, Lanes<8>& acc, const Lanes<8>& me,
+                                               const float (&nm)[8], float eps2, unsigned long long mask)
+{
+    float dx, dy, dz, s, t;
+    if constexpr (EQ == false && T == 0 && MODE == 0) asm volatile( NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 0 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 0 && MODE == 2) asm volatile(NBODY_SYM_PAIR_M(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 1 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 1 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 1 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR_M(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 2 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 2 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 2 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR_M(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 3 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 3 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 3 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR_M(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 4 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 4 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 4 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR_M(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 5 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 5 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 5 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR_M(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 6 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 6 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 6 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR_M(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 7 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 7 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == false && T == 7 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR_M(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 0 && MODE == 0) asm volatile( NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 0 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 0 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ_M(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 1 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 1 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 1 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ_M(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 2 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 2 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 2 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ_M(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 3 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 3 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 3 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ_M(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 4 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 4 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 4 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ_M(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 5 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 5 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 5 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ_M(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 6 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 6 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 6 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ_M(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 7 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 7 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    else if constexpr (EQ == true && T == 7 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ_M(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+}
+
+#undef NBODY_DIAG_OPS
+#undef NBODY_SYM_PAIR_M
+#undef NBODY_SYM_PAIR_EQ_M
 
 #undef NBODY_SYM_PAIR_EQ
 #undef NBODY_SYM_PAIR
@@ -362,6 +491,123 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
 #pragma unroll
     for (int u = 0; u < A; ++u) {
         const size_t o = (size_t)Q * npad + (size_t)P * B + u * 64 * W + threadIdx.x;
+        part[o] = acc.x[u] * scale;
+        part[plane + o] = acc.y[u] * scale;
+        part[2 * plane + o] = acc.z[u] * scale;
+    }
+}
+
+// Block P against itself, each pair once, on one wave with 8 bodies per lane
+// (blocks of 512). Tile t of the block rotates past the lanes as in a block
+// pair and meets the i sub-blocks u < t in full, both directions. Against its
+// own sub-block (u = t) it takes steps 1..32 only, step 32 on lanes 32..63:
+// a pair of lanes at rotation distance d is met at step d and at step 64 - d,
+// so this covers every pair of the 64 exactly once. When the tile's
+// reactions come home after 64 steps they sit in the lanes that hold the
+// same bodies' i sums, and are added there: no LDS, no cross-lane traffic.
+template <int T, bool NEWTON, bool EQ>
+__device__ __forceinline__ void sym_diag_phases(const float4* __restrict__ pm, int n, float eps2, int P,
+                                                const Lanes<8>& me, const float (&nm)[8], Lanes<8>& acc)
+{
+    const int jbase = P * 512 + T * 64;  // wave-uniform
+    if (jbase >= n) return;              // the rest of the block is padding
+    const int lane = threadIdx.x & 63;
+    const float4 b = lane < n - jbase ? pm[jbase + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
+    float rx = 0.f, ry = 0.f, rz = 0.f;
+    if constexpr (!NEWTON) {
+        constexpr unsigned long long upper = 0xffffffff00000000ull;  // lanes 32..63
+        sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
+        for (int k = 1; k < 32; ++k) sym_diag_step8<T, 1, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
+        sym_diag_step8<T, 2, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
+        for (int k = 33; k < 64; ++k) sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
+    } else {
+        // the same steps in C++ (with the Newton step on the rsqrt): one pair
+        // of i body (u, lane) and the j body now in this lane; `keep` false
+        // clears the pair's weights
+        const auto pair = [&](int u, bool keep) {
+            const float dx = xj - me.x[u];
+            const float dy = yj - me.y[u];
+            const float dz = zj - me.z[u];
+            float d2 = __builtin_fmaf(dx, dx, eps2);
+            d2 = __builtin_fmaf(dy, dy, d2);
+            d2 = __builtin_fmaf(dz, dz, d2);
+            const float inv = rsqrt_f32<NEWTON>(d2);
+            float inv3 = (inv * inv) * inv;
+            if (!keep) inv3 = 0.f;
+            const float si = EQ ? inv3 : mj * inv3;
+            const float sj = EQ ? -inv3 : nm[u] * inv3;
+            acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
+            acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
+            acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
+            rx = __builtin_fmaf(dx, sj, rx);
+            ry = __builtin_fmaf(dy, sj, ry);
+            rz = __builtin_fmaf(dz, sj, rz);
+        };
+        const auto rotate = [&] {
+            xj = wave_ror1(xj);
+            yj = wave_ror1(yj);
+            zj = wave_ror1(zj);
+            if constexpr (!EQ) mj = wave_ror1(mj);
+            rx = wave_ror1(rx);
+            ry = wave_ror1(ry);
+            rz = wave_ror1(rz);
+        };
+        // step 0: the tile meets itself lane by lane (self pairs, skipped)
+#pragma unroll
+        for (int u = 0; u < T; ++u) pair(u, true);
+        rotate();
+        for (int k = 1; k < 32; ++k) {
+#pragma unroll
+            for (int u = 0; u <= T; ++u) pair(u, true);
+            rotate();
+        }
+#pragma unroll
+        for (int u = 0; u <= T; ++u) pair(u, u < T || lane >= 32);
+        rotate();
+        for (int k = 33; k < 64; ++k) {
+#pragma unroll
+            for (int u = 0; u < T; ++u) pair(u, true);
+            rotate();
+        }
+    }
+    acc.x[T] += rx;
+    acc.y[T] += ry;
+    acc.z[T] += rz;
+    if constexpr (T + 1 < 8) sym_diag_phases<T + 1, NEWTON, EQ>(pm, n, eps2, P, me, nm, acc);
+}
+
+// Block P's diagonal as one task (see sym_diag_phases), into slot P. Bodies
+// past n are zero-mass padding and store into padding no reduction reads.
+template <bool NEWTON>
+__device__ __forceinline__ void sym_diag_block(const float4* __restrict__ pm, int n, int npad, float eps2,
+                                               float* __restrict__ part, int P)
+{
+    constexpr int B = 512;
+    const int lane = threadIdx.x & 63;
+    const size_t plane = (size_t)(npad / B) * npad;
+    const float m0 = pm[P * B].w;
+    Lanes<8> me, acc;
+    float nm[8];
+    bool same = true;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int i = P * B + u * 64 + lane;
+        const float4 q = i < n ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        me.x[u] = q.x;
+        me.y[u] = q.y;
+        me.z[u] = q.z;
+        nm[u] = -q.w;
+        same &= (i < n) & (q.w == m0);
+    }
+    zero<8>(acc);
+    const bool eq = __builtin_amdgcn_ballot_w64(!same) == 0;
+    if (eq) sym_diag_phases<0, NEWTON, true>(pm, n, eps2, P, me, nm, acc);
+    else sym_diag_phases<0, NEWTON, false>(pm, n, eps2, P, me, nm, acc);
+    const float scale = eq ? m0 : 1.0f;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const size_t o = (size_t)P * npad + (size_t)P * B + u * 64 + lane;
         part[o] = acc.x[u] * scale;
         part[plane + o] = acc.y[u] * scale;
         part[2 * plane + o] = acc.z[u] * scale;
