@@ -222,6 +222,13 @@ __global__ __launch_bounds__(64 * W) void k_force_sym(SymParams p)
 {
     constexpr int B = 64 * A * W;
     __shared__ float jacc[3][B];
+    // Registers decide how many waves share a SIMD; LDS only how many
+    // workgroups share a CU, which the dispatcher may spread unevenly over its
+    // four SIMDs. With 8 bodies per lane on one wave the kernel needs 71
+    // VGPRs (7 waves per SIMD; 6.5 by LDS) but runs fastest at exactly 6 per
+    // SIMD (121 TFLOP/s credited at N = 98304 against 115), so it claims v79:
+    // 80 VGPRs, 6 waves on every SIMD.
+    if constexpr (A == 8 && W == 1) asm volatile("" ::: "v79");
     const int npairs = p.nb * (p.nb - 1) / 2;
     const int t = blockIdx.x;
     if (t < npairs) {

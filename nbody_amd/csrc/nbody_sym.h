@@ -59,7 +59,9 @@ __device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
 // four or more instructions after the last write of what they move. The
 // arithmetic and its order are exactly those of the generic loop in sym_pair
 // (same operations per accumulator, u in increasing order), so results are
-// bit-identical to it.
+// bit-identical to it. The i masses come in unnegated; the reaction weight
+// -m_i s^3 takes its sign from the source modifier, so no negated copies of
+// the masses need registers.
 #define NBODY_SYM_PAIR(u)                                                                              \
     "v_sub_f32 %[dx], %[xj], %[x" #u "]\n"                                                             \
     "v_sub_f32 %[dy], %[yj], %[y" #u "]\n"                                                             \
@@ -72,7 +74,7 @@ __device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
     "v_mul_f32 %[t], %[s], %[s]\n"                                                                     \
     "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
     "v_mul_f32 %[s], %[mj], %[t]\n"                                                                    \
-    "v_mul_f32 %[t], %[n" #u "], %[t]\n"                                                               \
+    "v_mul_f32 %[t], -%[n" #u "], %[t]\n"                                                               \
     "v_fmac_f32 %[rx], %[dx], %[t]\n"                                                                  \
     "v_fmac_f32 %[ry], %[dy], %[t]\n"                                                                  \
     "v_fmac_f32 %[rz], %[dz], %[t]\n"                                                                  \
@@ -83,7 +85,7 @@ __device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
 #define NBODY_ROR1(r) "v_mov_b32_dpp %[" #r "], %[" #r "] wave_ror:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
 
 __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
-                                          Lanes<8>& acc, const Lanes<8>& me, const float (&nm)[8], float eps2)
+                                          Lanes<8>& acc, const Lanes<8>& me, const float (&mi)[8], float eps2)
 {
     float dx, dy, dz, s, t;
     asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
@@ -105,8 +107,8 @@ __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float
                    [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
                    [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
                    [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]),
-                   [n0] "v"(nm[0]), [n1] "v"(nm[1]), [n2] "v"(nm[2]), [n3] "v"(nm[3]),
-                   [n4] "v"(nm[4]), [n5] "v"(nm[5]), [n6] "v"(nm[6]), [n7] "v"(nm[7]), [eps] "s"(eps2));
+                   [n0] "v"(mi[0]), [n1] "v"(mi[1]), [n2] "v"(mi[2]), [n3] "v"(mi[3]),
+                   [n4] "v"(mi[4]), [n5] "v"(mi[5]), [n6] "v"(mi[6]), [n7] "v"(mi[7]), [eps] "s"(eps2));
 }
 
 // The same step for a block pair whose bodies all have one mass m0: the pair
@@ -150,7 +152,7 @@ __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float
     "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
     "v_cndmask_b32 %[t], 0, %[t], %[mask]\n"                                                           \
     "v_mul_f32 %[s], %[mj], %[t]\n"                                                                    \
-    "v_mul_f32 %[t], %[n" #u "], %[t]\n"                                                               \
+    "v_mul_f32 %[t], -%[n" #u "], %[t]\n"                                                               \
     "v_fmac_f32 %[rx], %[dx], %[t]\n"                                                                  \
     "v_fmac_f32 %[ry], %[dy], %[t]\n"                                                                  \
     "v_fmac_f32 %[rz], %[dz], %[t]\n"                                                                  \
@@ -204,7 +206,7 @@ __device__ __forceinline__ void sym_step8_eq(float& xj, float& yj, float& zj, fl
 
 // The same steps for 4 and 2 bodies per lane (the smaller block shapes).
 __device__ __forceinline__ void sym_step4(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
-                                          Lanes<4>& acc, const Lanes<4>& me, const float (&nm)[4], float eps2)
+                                          Lanes<4>& acc, const Lanes<4>& me, const float (&mi)[4], float eps2)
 {
     float dx, dy, dz, s, t;
     asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
@@ -218,8 +220,8 @@ __device__ __forceinline__ void sym_step4(float& xj, float& yj, float& zj, float
                    [t] "=&v"(t)
                  : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]), [y0] "v"(me.y[0]),
                    [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]), [z0] "v"(me.z[0]), [z1] "v"(me.z[1]),
-                   [z2] "v"(me.z[2]), [z3] "v"(me.z[3]), [n0] "v"(nm[0]), [n1] "v"(nm[1]), [n2] "v"(nm[2]),
-                   [n3] "v"(nm[3]), [eps] "s"(eps2));
+                   [z2] "v"(me.z[2]), [z3] "v"(me.z[3]), [n0] "v"(mi[0]), [n1] "v"(mi[1]), [n2] "v"(mi[2]),
+                   [n3] "v"(mi[3]), [eps] "s"(eps2));
 }
 
 __device__ __forceinline__ void sym_step4_eq(float& xj, float& yj, float& zj, float& rx, float& ry, float& rz,
@@ -240,7 +242,7 @@ __device__ __forceinline__ void sym_step4_eq(float& xj, float& yj, float& zj, fl
 }
 
 __device__ __forceinline__ void sym_step2(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
-                                          Lanes<2>& acc, const Lanes<2>& me, const float (&nm)[2], float eps2)
+                                          Lanes<2>& acc, const Lanes<2>& me, const float (&mi)[2], float eps2)
 {
     float dx, dy, dz, s, t;
     asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1)
@@ -251,7 +253,7 @@ __device__ __forceinline__ void sym_step2(float& xj, float& yj, float& zj, float
                    [ay1] "+v"(acc.y[1]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [dx] "=&v"(dx),
                    [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s), [t] "=&v"(t)
                  : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [z0] "v"(me.z[0]),
-                   [z1] "v"(me.z[1]), [n0] "v"(nm[0]), [n1] "v"(nm[1]), [eps] "s"(eps2));
+                   [z1] "v"(me.z[1]), [n0] "v"(mi[0]), [n1] "v"(mi[1]), [eps] "s"(eps2));
 }
 
 __device__ __forceinline__ void sym_step2_eq(float& xj, float& yj, float& zj, float& rx, float& ry, float& rz,
@@ -289,14 +291,14 @@ __device__ __forceinline__ void sym_step2_eq(float& xj, float& yj, float& zj, fl
       [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]), [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), \
       [y2] "v"(me.y[2]), [y3] "v"(me.y[3]), [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), \
       [y7] "v"(me.y[7]), [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]), \
-      [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]), [n0] "v"(nm[0]), \
-      [n1] "v"(nm[1]), [n2] "v"(nm[2]), [n3] "v"(nm[3]), [n4] "v"(nm[4]), [n5] "v"(nm[5]), [n6] "v"(nm[6]), \
-      [n7] "v"(nm[7]), [eps] "s"(eps2), [mask] "s"(mask)
+      [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]), [n0] "v"(mi[0]), \
+      [n1] "v"(mi[1]), [n2] "v"(mi[2]), [n3] "v"(mi[3]), [n4] "v"(mi[4]), [n5] "v"(mi[5]), [n6] "v"(mi[6]), \
+      [n7] "v"(mi[7]), [eps] "s"(eps2), [mask] "s"(mask)
 
 template <int T, int MODE, bool EQ>
 __device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry,
                                                float& rz, Lanes<8>& acc, const Lanes<8>& me,
-                                               const float (&nm)[8], float eps2, unsigned long long mask)
+                                               const float (&mi)[8], float eps2, unsigned long long mask)
 {
     float dx, dy, dz, s, t;
     if constexpr (EQ == false && T == 0 && MODE == 0) asm volatile( NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
@@ -362,7 +364,7 @@ __device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, 
 // reaction into jacc. With EQ the pair weights leave out the (common) mass.
 template <int A, int W, bool NEWTON, bool EQ>
 __device__ __forceinline__ void sym_phases(const float4* __restrict__ pm, int n, float eps2, int Q,
-                                           const Lanes<A>& me, const float (&nm)[A], Lanes<A>& acc,
+                                           const Lanes<A>& me, const float (&mi)[A], Lanes<A>& acc,
                                            float (*jacc)[64 * A * W])
 {
     constexpr int B = 64 * A * W;
@@ -384,17 +386,17 @@ __device__ __forceinline__ void sym_phases(const float4* __restrict__ pm, int n,
             for (int k = 0; k < 64; ++k) {
                 if constexpr (A == 8 && !NEWTON) {
                     if constexpr (EQ) sym_step8_eq(xj, yj, zj, rx, ry, rz, acc, me, eps2);
-                    else sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
+                    else sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2);
                     continue;
                 }
                 if constexpr (A == 4 && !NEWTON) {
                     if constexpr (EQ) sym_step4_eq(xj, yj, zj, rx, ry, rz, acc, me, eps2);
-                    else sym_step4(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
+                    else sym_step4(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2);
                     continue;
                 }
                 if constexpr (A == 2 && !NEWTON) {
                     if constexpr (EQ) sym_step2_eq(xj, yj, zj, rx, ry, rz, acc, me, eps2);
-                    else sym_step2(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2);
+                    else sym_step2(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2);
                     continue;
                 }
 #pragma unroll
@@ -408,7 +410,7 @@ __device__ __forceinline__ void sym_phases(const float4* __restrict__ pm, int n,
                     const float inv = rsqrt_f32<NEWTON>(d2);
                     const float inv3 = (inv * inv) * inv;
                     const float si = EQ ? inv3 : mj * inv3;
-                    const float sj = EQ ? -inv3 : nm[u] * inv3;
+                    const float sj = EQ ? -inv3 : -mi[u] * inv3;
                     acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
                     acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
                     acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
@@ -457,7 +459,7 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
     const float m0 = pm[P * B].w;
 
     Lanes<A> me, acc;
-    float nm[A];
+    float mi[A];
     bool same = true;
 #pragma unroll
     for (int u = 0; u < A; ++u) {
@@ -465,7 +467,7 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
         me.x[u] = q.x;
         me.y[u] = q.y;
         me.z[u] = q.z;
-        nm[u] = -q.w;
+        mi[u] = q.w;
         same &= q.w == m0;
     }
     // all A loads in flight at once (no short-circuit), clamped into the array
@@ -479,8 +481,8 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
     for (int k = threadIdx.x; k < B; k += 64 * W) jacc[0][k] = jacc[1][k] = jacc[2][k] = 0.f;
     __syncthreads();
     const bool eq = workgroup_all<W>(same);
-    if (eq) sym_phases<A, W, NEWTON, true>(pm, n, eps2, Q, me, nm, acc, jacc);
-    else sym_phases<A, W, NEWTON, false>(pm, n, eps2, Q, me, nm, acc, jacc);
+    if (eq) sym_phases<A, W, NEWTON, true>(pm, n, eps2, Q, me, mi, acc, jacc);
+    else sym_phases<A, W, NEWTON, false>(pm, n, eps2, Q, me, mi, acc, jacc);
     const float scale = eq ? m0 : 1.0f;
     for (int k = threadIdx.x; k < B; k += 64 * W) {
         const size_t o = (size_t)P * npad + (size_t)Q * B + k;
@@ -507,7 +509,7 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
 // same bodies' i sums, and are added there: no LDS, no cross-lane traffic.
 template <int T, bool NEWTON, bool EQ>
 __device__ __forceinline__ void sym_diag_phases(const float4* __restrict__ pm, int n, float eps2, int P,
-                                                const Lanes<8>& me, const float (&nm)[8], Lanes<8>& acc)
+                                                const Lanes<8>& me, const float (&mi)[8], Lanes<8>& acc)
 {
     const int jbase = P * 512 + T * 64;  // wave-uniform
     if (jbase >= n) return;              // the rest of the block is padding
@@ -517,10 +519,10 @@ __device__ __forceinline__ void sym_diag_phases(const float4* __restrict__ pm, i
     float rx = 0.f, ry = 0.f, rz = 0.f;
     if constexpr (!NEWTON) {
         constexpr unsigned long long upper = 0xffffffff00000000ull;  // lanes 32..63
-        sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
-        for (int k = 1; k < 32; ++k) sym_diag_step8<T, 1, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
-        sym_diag_step8<T, 2, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
-        for (int k = 33; k < 64; ++k) sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, nm, eps2, upper);
+        sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
+        for (int k = 1; k < 32; ++k) sym_diag_step8<T, 1, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
+        sym_diag_step8<T, 2, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
+        for (int k = 33; k < 64; ++k) sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
     } else {
         // the same steps in C++ (with the Newton step on the rsqrt): one pair
         // of i body (u, lane) and the j body now in this lane; `keep` false
@@ -536,7 +538,7 @@ __device__ __forceinline__ void sym_diag_phases(const float4* __restrict__ pm, i
             float inv3 = (inv * inv) * inv;
             if (!keep) inv3 = 0.f;
             const float si = EQ ? inv3 : mj * inv3;
-            const float sj = EQ ? -inv3 : nm[u] * inv3;
+            const float sj = EQ ? -inv3 : -mi[u] * inv3;
             acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
             acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
             acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
@@ -574,7 +576,7 @@ __device__ __forceinline__ void sym_diag_phases(const float4* __restrict__ pm, i
     acc.x[T] += rx;
     acc.y[T] += ry;
     acc.z[T] += rz;
-    if constexpr (T + 1 < 8) sym_diag_phases<T + 1, NEWTON, EQ>(pm, n, eps2, P, me, nm, acc);
+    if constexpr (T + 1 < 8) sym_diag_phases<T + 1, NEWTON, EQ>(pm, n, eps2, P, me, mi, acc);
 }
 
 // Block P's diagonal as one task (see sym_diag_phases), into slot P. Bodies
@@ -588,7 +590,7 @@ __device__ __forceinline__ void sym_diag_block(const float4* __restrict__ pm, in
     const size_t plane = (size_t)(npad / B) * npad;
     const float m0 = pm[P * B].w;
     Lanes<8> me, acc;
-    float nm[8];
+    float mi[8];
     bool same = true;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -597,13 +599,13 @@ __device__ __forceinline__ void sym_diag_block(const float4* __restrict__ pm, in
         me.x[u] = q.x;
         me.y[u] = q.y;
         me.z[u] = q.z;
-        nm[u] = -q.w;
+        mi[u] = q.w;
         same &= (i < n) & (q.w == m0);
     }
     zero<8>(acc);
     const bool eq = __builtin_amdgcn_ballot_w64(!same) == 0;
-    if (eq) sym_diag_phases<0, NEWTON, true>(pm, n, eps2, P, me, nm, acc);
-    else sym_diag_phases<0, NEWTON, false>(pm, n, eps2, P, me, nm, acc);
+    if (eq) sym_diag_phases<0, NEWTON, true>(pm, n, eps2, P, me, mi, acc);
+    else sym_diag_phases<0, NEWTON, false>(pm, n, eps2, P, me, mi, acc);
     const float scale = eq ? m0 : 1.0f;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
