@@ -52,18 +52,19 @@ enum {
 
 /* Options of the general entry; zero / -1 fields mean "choose".
  * For NBODY_HIP_SYMMETRIC, i_per_lane is 2, 4 or 8 and jslice is the block
- * size B = 64 * i_per_lane * {1, 2, 4} (2 only with 4); its block slots are
+ * size B = 64 * i_per_lane * {1, 2, 4} (2 with 1 or 4); its block slots are
  * always summed with TwoSum, and it computes whole systems only (base 0, n N).
- * waves_per_simd caps how many of its waves share a SIMD (the launch reserves
- * extra LDS per workgroup); how the block-pair tasks divide into the resident
- * waves changes how evenly the launch ends. */
+ * waves_per_simd caps its resident waves at about that many per SIMD: the
+ * launch reserves extra LDS per workgroup, which bounds the workgroups per CU
+ * (the dispatcher may still load the four SIMDs unevenly); how the block-pair
+ * tasks divide into the resident waves changes how evenly the launch ends. */
 typedef struct nbody_hip_options {
     int variant;        /* NBODY_HIP_* or NBODY_HIP_AUTO */
     int i_per_lane;     /* 1, 2, 4 (symmetric 2, 4, 8), or 0 = auto */
     int jslice;         /* j bodies per slice (symmetric: block size), or 0 = auto */
     int compensated;    /* 1 TwoSum-compensated sums, 0 plain, -1 auto (on for N > 65536) */
     int newton;         /* 1 adds a Newton step after v_rsq_f32 */
-    int waves_per_simd; /* symmetric only: 1..8, or 0 = as many as the kernel's registers allow */
+    int waves_per_simd; /* symmetric only: 1..8, or 0 = as many as registers and LDS allow */
 } nbody_hip_options;
 
 /* ---- algorithm-table entries (AOS_GPU contract above) -------------------- */
