@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kWave) void k_force_batch(const ForceTask* __restri
             continue;
         }
         if (task.kind == TASK_DIAG) {
-            sym_diag_block<false>(task.pm, task.n, task.npad, task.eps2, task.part, task.a);
+            sym_diag_block(task.pm, task.n, task.npad, task.eps2, task.part, task.a);
             continue;
         }
         // TASK_ROW_COMP: 64 bodies against task.len bodies, TwoSum-folded chunks

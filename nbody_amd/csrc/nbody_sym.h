@@ -507,7 +507,7 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
 // so this covers every pair of the 64 exactly once. When the tile's
 // reactions come home after 64 steps they sit in the lanes that hold the
 // same bodies' i sums, and are added there: no LDS, no cross-lane traffic.
-template <int T, bool NEWTON, bool EQ>
+template <int T, bool EQ>
 __device__ __forceinline__ void sym_diag_phases(const float4* __restrict__ pm, int n, float eps2, int P,
                                                 const Lanes<8>& me, const float (&mi)[8], Lanes<8>& acc)
 {
@@ -517,71 +517,21 @@ __device__ __forceinline__ void sym_diag_phases(const float4* __restrict__ pm, i
     const float4 b = lane < n - jbase ? pm[jbase + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
     float rx = 0.f, ry = 0.f, rz = 0.f;
-    if constexpr (!NEWTON) {
-        constexpr unsigned long long upper = 0xffffffff00000000ull;  // lanes 32..63
-        sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
-        for (int k = 1; k < 32; ++k) sym_diag_step8<T, 1, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
-        sym_diag_step8<T, 2, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
-        for (int k = 33; k < 64; ++k) sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
-    } else {
-        // the same steps in C++ (with the Newton step on the rsqrt): one pair
-        // of i body (u, lane) and the j body now in this lane; `keep` false
-        // clears the pair's weights
-        const auto pair = [&](int u, bool keep) {
-            const float dx = xj - me.x[u];
-            const float dy = yj - me.y[u];
-            const float dz = zj - me.z[u];
-            float d2 = __builtin_fmaf(dx, dx, eps2);
-            d2 = __builtin_fmaf(dy, dy, d2);
-            d2 = __builtin_fmaf(dz, dz, d2);
-            const float inv = rsqrt_f32<NEWTON>(d2);
-            float inv3 = (inv * inv) * inv;
-            if (!keep) inv3 = 0.f;
-            const float si = EQ ? inv3 : mj * inv3;
-            const float sj = EQ ? -inv3 : -mi[u] * inv3;
-            acc.x[u] = __builtin_fmaf(dx, si, acc.x[u]);
-            acc.y[u] = __builtin_fmaf(dy, si, acc.y[u]);
-            acc.z[u] = __builtin_fmaf(dz, si, acc.z[u]);
-            rx = __builtin_fmaf(dx, sj, rx);
-            ry = __builtin_fmaf(dy, sj, ry);
-            rz = __builtin_fmaf(dz, sj, rz);
-        };
-        const auto rotate = [&] {
-            xj = wave_ror1(xj);
-            yj = wave_ror1(yj);
-            zj = wave_ror1(zj);
-            if constexpr (!EQ) mj = wave_ror1(mj);
-            rx = wave_ror1(rx);
-            ry = wave_ror1(ry);
-            rz = wave_ror1(rz);
-        };
-        // step 0: the tile meets itself lane by lane (self pairs, skipped)
-#pragma unroll
-        for (int u = 0; u < T; ++u) pair(u, true);
-        rotate();
-        for (int k = 1; k < 32; ++k) {
-#pragma unroll
-            for (int u = 0; u <= T; ++u) pair(u, true);
-            rotate();
-        }
-#pragma unroll
-        for (int u = 0; u <= T; ++u) pair(u, u < T || lane >= 32);
-        rotate();
-        for (int k = 33; k < 64; ++k) {
-#pragma unroll
-            for (int u = 0; u < T; ++u) pair(u, true);
-            rotate();
-        }
-    }
+    constexpr unsigned long long upper = 0xffffffff00000000ull;  // lanes 32..63
+    // step 0: the tile meets itself lane by lane (self pairs, skipped)
+    sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
+    for (int k = 1; k < 32; ++k) sym_diag_step8<T, 1, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
+    sym_diag_step8<T, 2, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
+    for (int k = 33; k < 64; ++k) sym_diag_step8<T, 0, EQ>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2, upper);
     acc.x[T] += rx;
     acc.y[T] += ry;
     acc.z[T] += rz;
-    if constexpr (T + 1 < 8) sym_diag_phases<T + 1, NEWTON, EQ>(pm, n, eps2, P, me, mi, acc);
+    if constexpr (T + 1 < 8) sym_diag_phases<T + 1, EQ>(pm, n, eps2, P, me, mi, acc);
 }
 
 // Block P's diagonal as one task (see sym_diag_phases), into slot P. Bodies
 // past n are zero-mass padding and store into padding no reduction reads.
-template <bool NEWTON>
+// (v_rsq_f32 without a Newton step, as the batched kernel uses it.)
 __device__ __forceinline__ void sym_diag_block(const float4* __restrict__ pm, int n, int npad, float eps2,
                                                float* __restrict__ part, int P)
 {
@@ -604,8 +554,8 @@ __device__ __forceinline__ void sym_diag_block(const float4* __restrict__ pm, in
     }
     zero<8>(acc);
     const bool eq = __builtin_amdgcn_ballot_w64(!same) == 0;
-    if (eq) sym_diag_phases<0, NEWTON, true>(pm, n, eps2, P, me, mi, acc);
-    else sym_diag_phases<0, NEWTON, false>(pm, n, eps2, P, me, mi, acc);
+    if (eq) sym_diag_phases<0, true>(pm, n, eps2, P, me, mi, acc);
+    else sym_diag_phases<0, false>(pm, n, eps2, P, me, mi, acc);
     const float scale = eq ? m0 : 1.0f;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
