@@ -84,31 +84,64 @@ __device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
 
 #define NBODY_ROR1(r) "v_mov_b32_dpp %[" #r "], %[" #r "] wave_ror:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
 
+// The chain's five temporaries (d, then s, then t = s^3) are either left to
+// the compiler or, with PIN, held in v21, v25, v29, v33 and v34. On gfx950
+// the issue rate of one instruction stream depends on which registers hold
+// them relative to the rest: the loop bodies run verbatim, ALU only, issue
+// 1.18 to 1.29 VALU instructions per clock per CU. The single-system kernel's
+// own allocation measures at the top of that range. In the batched kernel
+// the compiler picks a slow one, and pinning raises that kernel's rate by
+// about 7 %.
+template <bool PIN>
 __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
                                           Lanes<8>& acc, const Lanes<8>& me, const float (&mi)[8], float eps2)
 {
     float dx, dy, dz, s, t;
-    asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
-                 NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR(7)
-                 NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj)
-                 NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
-                 : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry),
-                   [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
-                   [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
-                   [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
-                   [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
-                   [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
-                   [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
-                   [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s),
-                   [t] "=&v"(t)
-                 : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
-                   [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
-                   [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
-                   [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
-                   [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
-                   [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]),
-                   [n0] "v"(mi[0]), [n1] "v"(mi[1]), [n2] "v"(mi[2]), [n3] "v"(mi[3]),
-                   [n4] "v"(mi[4]), [n5] "v"(mi[5]), [n6] "v"(mi[6]), [n7] "v"(mi[7]), [eps] "s"(eps2));
+    if constexpr (PIN) {
+        asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
+                     NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR(7)
+                     NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj)
+                     NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry),
+                       [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                       [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
+                       [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
+                       [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
+                       [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
+                       [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
+                       [az7] "+v"(acc.z[7]), [dx] "=&{v21}"(dx), [dy] "=&{v25}"(dy), [dz] "=&{v29}"(dz),
+                       [s] "=&{v33}"(s), [t] "=&{v34}"(t)
+                     : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
+                       [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
+                       [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
+                       [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
+                       [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
+                       [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]),
+                       [n0] "v"(mi[0]), [n1] "v"(mi[1]), [n2] "v"(mi[2]), [n3] "v"(mi[3]),
+                       [n4] "v"(mi[4]), [n5] "v"(mi[5]), [n6] "v"(mi[6]), [n7] "v"(mi[7]), [eps] "s"(eps2));
+    } else {
+        asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
+                     NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR(7)
+                     NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj)
+                     NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry),
+                       [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                       [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
+                       [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
+                       [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
+                       [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
+                       [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
+                       [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s),
+                       [t] "=&v"(t)
+                     : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
+                       [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
+                       [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
+                       [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
+                       [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
+                       [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]),
+                       [n0] "v"(mi[0]), [n1] "v"(mi[1]), [n2] "v"(mi[2]), [n3] "v"(mi[3]),
+                       [n4] "v"(mi[4]), [n5] "v"(mi[5]), [n6] "v"(mi[6]), [n7] "v"(mi[7]), [eps] "s"(eps2));
+    }
 }
 
 // The same step for a block pair whose bodies all have one mass m0: the pair
@@ -179,29 +212,52 @@ __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float
     "v_fmac_f32 %[ay" #u "], %[dy], %[t]\n"                                                            \
     "v_fmac_f32 %[az" #u "], %[dz], %[t]\n"
 
+template <bool PIN>
 __device__ __forceinline__ void sym_step8_eq(float& xj, float& yj, float& zj, float& rx, float& ry, float& rz,
                                              Lanes<8>& acc, const Lanes<8>& me, float eps2)
 {
     float dx, dy, dz, s, t;
-    asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3)
-                 NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ(7)
-                 NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj)
-                 NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
-                 : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [rx] "+v"(rx), [ry] "+v"(ry),
-                   [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
-                   [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
-                   [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
-                   [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
-                   [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
-                   [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
-                   [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s),
-                   [t] "=&v"(t)
-                 : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
-                   [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
-                   [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
-                   [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
-                   [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
-                   [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]), [eps] "s"(eps2));
+    if constexpr (PIN) {
+        asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3)
+                     NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ(7)
+                     NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj)
+                     NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [rx] "+v"(rx), [ry] "+v"(ry),
+                       [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                       [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
+                       [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
+                       [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
+                       [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
+                       [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
+                       [az7] "+v"(acc.z[7]), [dx] "=&{v21}"(dx), [dy] "=&{v25}"(dy), [dz] "=&{v29}"(dz),
+                       [s] "=&{v33}"(s), [t] "=&{v34}"(t)
+                     : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
+                       [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
+                       [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
+                       [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
+                       [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
+                       [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]), [eps] "s"(eps2));
+    } else {
+        asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3)
+                     NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ(7)
+                     NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj)
+                     NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [rx] "+v"(rx), [ry] "+v"(ry),
+                       [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                       [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
+                       [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
+                       [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
+                       [ay7] "+v"(acc.y[7]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
+                       [az3] "+v"(acc.z[3]), [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]),
+                       [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s),
+                       [t] "=&v"(t)
+                     : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]),
+                       [x4] "v"(me.x[4]), [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]),
+                       [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]),
+                       [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), [y7] "v"(me.y[7]),
+                       [z0] "v"(me.z[0]), [z1] "v"(me.z[1]), [z2] "v"(me.z[2]), [z3] "v"(me.z[3]),
+                       [z4] "v"(me.z[4]), [z5] "v"(me.z[5]), [z6] "v"(me.z[6]), [z7] "v"(me.z[7]), [eps] "s"(eps2));
+    }
 }
 
 // The same steps for 4 and 2 bodies per lane (the smaller block shapes).
@@ -276,8 +332,9 @@ __device__ __forceinline__ void sym_step2_eq(float& xj, float& yj, float& zj, fl
 // (MODE 1) the tile's own sub-block, or (MODE 2) the same with the weights of
 // the lanes outside `mask` cleared; then the rotation. Every operand of
 // sym_step8 is bound (they are live through the task anyway), so the
-// registers stay those of the block-pair step. With EQ the mass neither
-// enters nor rotates.
+// registers stay those of the block-pair step; the temporaries are pinned as
+// in sym_step8<true> (the batched kernel is the only user). With EQ the mass
+// neither enters nor rotates.
 #define NBODY_DIAG_OPS \
     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry), [rz] "+v"(rz), \
       [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]), [ax3] "+v"(acc.x[3]), \
@@ -285,8 +342,8 @@ __device__ __forceinline__ void sym_step2_eq(float& xj, float& yj, float& zj, fl
       [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]), [ay3] "+v"(acc.y[3]), \
       [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]), [ay7] "+v"(acc.y[7]), \
       [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]), [az3] "+v"(acc.z[3]), \
-      [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]), [az7] "+v"(acc.z[7]), [dx] "=&v"(dx), \
-      [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s), [t] "=&v"(t) \
+      [az4] "+v"(acc.z[4]), [az5] "+v"(acc.z[5]), [az6] "+v"(acc.z[6]), [az7] "+v"(acc.z[7]), [dx] "=&{v21}"(dx), \
+      [dy] "=&{v25}"(dy), [dz] "=&{v29}"(dz), [s] "=&{v33}"(s), [t] "=&{v34}"(t) \
     : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]), [x4] "v"(me.x[4]), \
       [x5] "v"(me.x[5]), [x6] "v"(me.x[6]), [x7] "v"(me.x[7]), [y0] "v"(me.y[0]), [y1] "v"(me.y[1]), \
       [y2] "v"(me.y[2]), [y3] "v"(me.y[3]), [y4] "v"(me.y[4]), [y5] "v"(me.y[5]), [y6] "v"(me.y[6]), \
@@ -362,7 +419,7 @@ __device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, 
 // The phases of block pair (P, Q): every wave streams the T tiles of block Q
 // past its A i bodies per lane, adding the i side into acc and each tile's
 // reaction into jacc. With EQ the pair weights leave out the (common) mass.
-template <int A, int W, bool NEWTON, bool EQ>
+template <int A, int W, bool NEWTON, bool EQ, bool PIN>
 __device__ __forceinline__ void sym_phases(const float4* __restrict__ pm, int n, float eps2, int Q,
                                            const Lanes<A>& me, const float (&mi)[A], Lanes<A>& acc,
                                            float (*jacc)[64 * A * W])
@@ -385,8 +442,8 @@ __device__ __forceinline__ void sym_phases(const float4* __restrict__ pm, int n,
             float rx = 0.f, ry = 0.f, rz = 0.f;
             for (int k = 0; k < 64; ++k) {
                 if constexpr (A == 8 && !NEWTON) {
-                    if constexpr (EQ) sym_step8_eq(xj, yj, zj, rx, ry, rz, acc, me, eps2);
-                    else sym_step8(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2);
+                    if constexpr (EQ) sym_step8_eq<PIN>(xj, yj, zj, rx, ry, rz, acc, me, eps2);
+                    else sym_step8<PIN>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2);
                     continue;
                 }
                 if constexpr (A == 4 && !NEWTON) {
@@ -450,7 +507,7 @@ __device__ __forceinline__ bool workgroup_all(bool ok)
 // weights leave the mass out and the partials are multiplied by it once, at
 // the store; with m = 1 (or any power of two) the result is bit-identical to
 // the general weights.
-template <int A, int W, bool NEWTON>
+template <int A, int W, bool NEWTON, bool PIN = false>
 __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, int npad, float eps2,
                                          float* __restrict__ part, int P, int Q, float (*jacc)[64 * A * W])
 {
@@ -481,8 +538,8 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
     for (int k = threadIdx.x; k < B; k += 64 * W) jacc[0][k] = jacc[1][k] = jacc[2][k] = 0.f;
     __syncthreads();
     const bool eq = workgroup_all<W>(same);
-    if (eq) sym_phases<A, W, NEWTON, true>(pm, n, eps2, Q, me, mi, acc, jacc);
-    else sym_phases<A, W, NEWTON, false>(pm, n, eps2, Q, me, mi, acc, jacc);
+    if (eq) sym_phases<A, W, NEWTON, true, PIN>(pm, n, eps2, Q, me, mi, acc, jacc);
+    else sym_phases<A, W, NEWTON, false, PIN>(pm, n, eps2, Q, me, mi, acc, jacc);
     const float scale = eq ? m0 : 1.0f;
     for (int k = threadIdx.x; k < B; k += 64 * W) {
         const size_t o = (size_t)P * npad + (size_t)Q * B + k;
