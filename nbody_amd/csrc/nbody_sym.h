@@ -352,63 +352,134 @@ __device__ __forceinline__ void sym_step2_eq(float& xj, float& yj, float& zj, fl
       [n1] "v"(mi[1]), [n2] "v"(mi[2]), [n3] "v"(mi[3]), [n4] "v"(mi[4]), [n5] "v"(mi[5]), [n6] "v"(mi[6]), \
       [n7] "v"(mi[7]), [eps] "s"(eps2), [mask] "s"(mask)
 
+// NBODY_LT<k>(P) expands to P(0) ... P(k-1): the pairs with the sub-blocks below tile k.
+#define NBODY_LT0(P)
+#define NBODY_LT1(P) NBODY_LT0(P) P(0)
+#define NBODY_LT2(P) NBODY_LT1(P) P(1)
+#define NBODY_LT3(P) NBODY_LT2(P) P(2)
+#define NBODY_LT4(P) NBODY_LT3(P) P(3)
+#define NBODY_LT5(P) NBODY_LT4(P) P(4)
+#define NBODY_LT6(P) NBODY_LT5(P) P(5)
+#define NBODY_LT7(P) NBODY_LT6(P) P(6)
+#define NBODY_ROR_GEN \
+    NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+#define NBODY_ROR_EQ NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
+
 template <int T, int MODE, bool EQ>
 __device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry,
                                                float& rz, Lanes<8>& acc, const Lanes<8>& me,
                                                const float (&mi)[8], float eps2, unsigned long long mask)
 {
     float dx, dy, dz, s, t;
-    if constexpr (EQ == false && T == 0 && MODE == 0) asm volatile( NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 0 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 0 && MODE == 2) asm volatile(NBODY_SYM_PAIR_M(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 1 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 1 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 1 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR_M(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 2 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 2 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 2 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR_M(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 3 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 3 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 3 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR_M(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 4 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 4 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 4 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR_M(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 5 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 5 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 5 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR_M(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 6 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 6 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 6 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR_M(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 7 && MODE == 0) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 7 && MODE == 1) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == false && T == 7 && MODE == 2) asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3) NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR_M(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 0 && MODE == 0) asm volatile( NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 0 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 0 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ_M(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 1 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 1 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 1 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ_M(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 2 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 2 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 2 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ_M(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 3 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 3 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 3 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ_M(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 4 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 4 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 4 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ_M(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 5 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 5 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 5 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ_M(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 6 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 6 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 6 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ_M(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 7 && MODE == 0) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 7 && MODE == 1) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
-    else if constexpr (EQ == true && T == 7 && MODE == 2) asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3) NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ_M(7) NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz) NBODY_DIAG_OPS);
+    if constexpr (!EQ && T == 0 && MODE == 0)
+        asm volatile(NBODY_LT0(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 0 && MODE == 1)
+        asm volatile(NBODY_LT0(NBODY_SYM_PAIR) NBODY_SYM_PAIR(0) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 0 && MODE == 2)
+        asm volatile(NBODY_LT0(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(0) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 1 && MODE == 0)
+        asm volatile(NBODY_LT1(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 1 && MODE == 1)
+        asm volatile(NBODY_LT1(NBODY_SYM_PAIR) NBODY_SYM_PAIR(1) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 1 && MODE == 2)
+        asm volatile(NBODY_LT1(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(1) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 2 && MODE == 0)
+        asm volatile(NBODY_LT2(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 2 && MODE == 1)
+        asm volatile(NBODY_LT2(NBODY_SYM_PAIR) NBODY_SYM_PAIR(2) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 2 && MODE == 2)
+        asm volatile(NBODY_LT2(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(2) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 3 && MODE == 0)
+        asm volatile(NBODY_LT3(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 3 && MODE == 1)
+        asm volatile(NBODY_LT3(NBODY_SYM_PAIR) NBODY_SYM_PAIR(3) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 3 && MODE == 2)
+        asm volatile(NBODY_LT3(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(3) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 4 && MODE == 0)
+        asm volatile(NBODY_LT4(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 4 && MODE == 1)
+        asm volatile(NBODY_LT4(NBODY_SYM_PAIR) NBODY_SYM_PAIR(4) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 4 && MODE == 2)
+        asm volatile(NBODY_LT4(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(4) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 5 && MODE == 0)
+        asm volatile(NBODY_LT5(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 5 && MODE == 1)
+        asm volatile(NBODY_LT5(NBODY_SYM_PAIR) NBODY_SYM_PAIR(5) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 5 && MODE == 2)
+        asm volatile(NBODY_LT5(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(5) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 6 && MODE == 0)
+        asm volatile(NBODY_LT6(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 6 && MODE == 1)
+        asm volatile(NBODY_LT6(NBODY_SYM_PAIR) NBODY_SYM_PAIR(6) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 6 && MODE == 2)
+        asm volatile(NBODY_LT6(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(6) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 7 && MODE == 0)
+        asm volatile(NBODY_LT7(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 7 && MODE == 1)
+        asm volatile(NBODY_LT7(NBODY_SYM_PAIR) NBODY_SYM_PAIR(7) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (!EQ && T == 7 && MODE == 2)
+        asm volatile(NBODY_LT7(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(7) NBODY_ROR_GEN NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 0 && MODE == 0)
+        asm volatile(NBODY_LT0(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 0 && MODE == 1)
+        asm volatile(NBODY_LT0(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(0) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 0 && MODE == 2)
+        asm volatile(NBODY_LT0(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(0) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 1 && MODE == 0)
+        asm volatile(NBODY_LT1(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 1 && MODE == 1)
+        asm volatile(NBODY_LT1(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(1) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 1 && MODE == 2)
+        asm volatile(NBODY_LT1(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(1) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 2 && MODE == 0)
+        asm volatile(NBODY_LT2(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 2 && MODE == 1)
+        asm volatile(NBODY_LT2(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(2) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 2 && MODE == 2)
+        asm volatile(NBODY_LT2(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(2) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 3 && MODE == 0)
+        asm volatile(NBODY_LT3(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 3 && MODE == 1)
+        asm volatile(NBODY_LT3(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(3) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 3 && MODE == 2)
+        asm volatile(NBODY_LT3(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(3) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 4 && MODE == 0)
+        asm volatile(NBODY_LT4(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 4 && MODE == 1)
+        asm volatile(NBODY_LT4(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(4) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 4 && MODE == 2)
+        asm volatile(NBODY_LT4(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(4) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 5 && MODE == 0)
+        asm volatile(NBODY_LT5(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 5 && MODE == 1)
+        asm volatile(NBODY_LT5(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(5) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 5 && MODE == 2)
+        asm volatile(NBODY_LT5(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(5) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 6 && MODE == 0)
+        asm volatile(NBODY_LT6(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 6 && MODE == 1)
+        asm volatile(NBODY_LT6(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(6) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 6 && MODE == 2)
+        asm volatile(NBODY_LT6(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(6) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 7 && MODE == 0)
+        asm volatile(NBODY_LT7(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 7 && MODE == 1)
+        asm volatile(NBODY_LT7(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(7) NBODY_ROR_EQ NBODY_DIAG_OPS);
+    else if constexpr (EQ && T == 7 && MODE == 2)
+        asm volatile(NBODY_LT7(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ_M(7) NBODY_ROR_EQ NBODY_DIAG_OPS);
 }
 
 #undef NBODY_DIAG_OPS
+#undef NBODY_LT0
+#undef NBODY_LT1
+#undef NBODY_LT2
+#undef NBODY_LT3
+#undef NBODY_LT4
+#undef NBODY_LT5
+#undef NBODY_LT6
+#undef NBODY_LT7
+#undef NBODY_ROR_GEN
+#undef NBODY_ROR_EQ
 #undef NBODY_SYM_PAIR_M
 #undef NBODY_SYM_PAIR_EQ_M
 
