@@ -20,6 +20,10 @@ diagnostic), and a CPU baseline: the reference's CPU_SOA_tiled
 (src/nbody_CPU_SOA_tiled.cpp:51-115) compiled from its sources when that
 build is present, else the oracle's restatement of it, timed on this host's
 cores on a bounded number of the same steps.
+
+Other workloads (--workload): ensemble (512 systems, one persistent launch
+per step), served (a Poisson request stream), plummer (N = 2^20), share (one
+GPU's i-range share of the N = 98304 system split over --shares GPUs).
 """
 import argparse
 import ctypes
@@ -49,10 +53,11 @@ def parse():
     ap.add_argument("--waves", type=int, default=0, help="pin the symmetric plan's waves per SIMD (skips the autotuner)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration for --pmc runs)")
-    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "served", "plummer"],
+    ap.add_argument("--workload", default="single", choices=["single", "ensemble", "served", "plummer", "share"],
                     help="single: one N-body system (the headline line); ensemble: 512 independent systems; "
                          "served: a Poisson stream of simulation requests; plummer: N = 2^20 Plummer sphere, "
                          "plain vs compensated sums")
+    ap.add_argument("--shares", type=int, default=8, help="share: GPUs the single system would be split over")
     ap.add_argument("--systems", type=int, default=512)
     ap.add_argument("--requests", type=int, default=400)
     ap.add_argument("--max-active", type=int, default=64)
@@ -319,6 +324,77 @@ def bench_plummer(args):
     }))
 
 
+def bench_share(args):
+    """One GPU's share of a single system split by i-range over --shares GPUs
+    (the multi-GPU decomposition of src/nbody_multiGPU.cu:87-124): bodies
+    [base, base + N/G) against all N through nbody_hip_force's base/n, the
+    one-sided plans timed over ipl and j-slice lengths, the fastest reported.
+    Positions would be all-gathered between steps; that exchange is not part
+    of this one-GPU measurement."""
+    import torch
+    from nbody_amd import _native, ics
+    L = _native.lib()
+    n, g = args.n, args.shares
+    if g < 2 or n % g:
+        raise SystemExit(f"--shares must be >= 2 and divide N (got {g}, N = {n})")
+    count = n // g
+    base = (g // 2) * count
+    eps2 = 0.01
+    pm_h, _ = ics.uniform_ball(n, seed=1)
+    dev = torch.device("cuda:0")
+    pm = torch.from_numpy(pm_h).to(dev)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    force = torch.zeros((count, 4), dtype=torch.float32, device=dev)
+    launches = max(args.steps, 5)
+    results = []
+    for variant in (_native.LDS, _native.SGPR, _native.DPP):
+        for ipl in (1, 2, 4):
+            opt = _native.Options.make(variant=variant, i_per_lane=ipl, compensated=1)
+            for _ in range(3):
+                _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, base, count, ctypes.byref(opt), None), "share")
+            torch.cuda.synchronize()
+            _native.check(L.nbody_hip_kernel_timer(1), "timer")
+            t0 = time.perf_counter()
+            for _ in range(launches):
+                _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, base, count, ctypes.byref(opt), None), "share")
+            torch.cuda.synchronize()
+            call_ms = (time.perf_counter() - t0) * 1e3 / launches
+            kms, nl = ctypes.c_double(), ctypes.c_int()
+            _native.check(L.nbody_hip_kernel_timer_read(ctypes.byref(kms), ctypes.byref(nl)), "timer read")
+            L.nbody_hip_kernel_timer(0)
+            plan = _native.Options()
+            _native.check(L.nbody_hip_plan(n, count, ctypes.byref(opt), ctypes.byref(plan)), "plan")
+            results.append((call_ms, kms.value / nl.value, plan))
+    ms, kernel_ms, plan = min(results, key=lambda r: r[0])
+    opt = _native.Options.make(variant=plan.variant, i_per_lane=plan.i_per_lane, jslice=plan.jslice,
+                               compensated=plan.compensated)
+    _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, base, count, ctypes.byref(opt), None), "share")
+    f64 = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    _native.check_ms(L.nbody_hip_force_fp64(P(f64), P(pm), eps2, n), "fp64")
+    ref = f64[base:base + count, :3]
+    d = (force[:, :3].double() - ref).norm(dim=1) / ref.norm(dim=1)
+    inter = float(count) * n
+    tflops = 20.0 * inter / (kernel_ms * 1e-3) / 1e12
+    print(json.dumps({
+        "metric": METRIC, "value": round(inter / (ms * 1e-3) / 1e9, 2), "unit": "10^9 interactions/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded uniform ball, unit masses)",
+        "config": {"workload": f"one GPU's share of N={n} split over {g} GPUs: bodies [{base}, {base + count}) "
+                               f"against all N (nbody_hip_force base/n), eps^2={eps2}; whole calls (force "
+                               f"kernel + slice sum), best of the one-sided plans over {launches} calls each"},
+        "ms_per_call": round(ms, 4),
+        "kernel_ms": round(kernel_ms, 4),
+        "roofline": {"bound": "valu", "achieved": round(tflops, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / PEAK_FP32_TFLOPS, 4), "kernel": KERNEL_NAMES[plan.variant]},
+        "plan": {"variant": _native.VARIANT_NAMES[plan.variant], "i_per_lane": plan.i_per_lane,
+                 "jslice": plan.jslice, "compensated": bool(plan.compensated)},
+        "accuracy": {"max_rel": float(d.max()), "rms_rel": float(d.pow(2).mean().sqrt()),
+                     "vs": "FP64 direct sum over the share's bodies"},
+        "all_plans_ms_per_call": {f"{_native.VARIANT_NAMES[p.variant]}/ipl{p.i_per_lane}/js{p.jslice}": round(t, 4)
+                                  for t, _, p in results},
+    }))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -349,6 +425,8 @@ def main():
         return bench_served(args)
     if args.workload == "plummer":
         return bench_plummer(args)
+    if args.workload == "share":
+        return bench_share(args)
 
     L = _native.lib()
     n, K, W = args.n, args.steps, args.warmup
