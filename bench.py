@@ -347,24 +347,32 @@ def bench_share(args):
     force = torch.zeros((count, 4), dtype=torch.float32, device=dev)
     launches = max(args.steps, 5)
     results = []
+    shapes = []
     for variant in (_native.LDS, _native.SGPR, _native.DPP):
         for ipl in (1, 2, 4):
-            opt = _native.Options.make(variant=variant, i_per_lane=ipl, compensated=1)
-            for _ in range(3):
-                _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, base, count, ctypes.byref(opt), None), "share")
-            torch.cuda.synchronize()
-            _native.check(L.nbody_hip_kernel_timer(1), "timer")
-            t0 = time.perf_counter()
-            for _ in range(launches):
-                _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, base, count, ctypes.byref(opt), None), "share")
-            torch.cuda.synchronize()
-            call_ms = (time.perf_counter() - t0) * 1e3 / launches
-            kms, nl = ctypes.c_double(), ctypes.c_int()
-            _native.check(L.nbody_hip_kernel_timer_read(ctypes.byref(kms), ctypes.byref(nl)), "timer read")
-            L.nbody_hip_kernel_timer(0)
-            plan = _native.Options()
-            _native.check(L.nbody_hip_plan(n, count, ctypes.byref(opt), ctypes.byref(plan)), "plan")
-            results.append((call_ms, kms.value / nl.value, plan))
+            auto = _native.Options()
+            o = _native.Options.make(variant=variant, i_per_lane=ipl, compensated=1)
+            _native.check(L.nbody_hip_plan(n, count, ctypes.byref(o), ctypes.byref(auto)), "plan")
+            for mul in (1, 2, 4, 8):
+                if mul == 1 or auto.jslice * mul <= n:
+                    shapes.append((variant, ipl, min(auto.jslice * mul, n)))
+    for variant, ipl, jslice in shapes:
+        opt = _native.Options.make(variant=variant, i_per_lane=ipl, jslice=jslice, compensated=1)
+        for _ in range(3):
+            _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, base, count, ctypes.byref(opt), None), "share")
+        torch.cuda.synchronize()
+        _native.check(L.nbody_hip_kernel_timer(1), "timer")
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, base, count, ctypes.byref(opt), None), "share")
+        torch.cuda.synchronize()
+        call_ms = (time.perf_counter() - t0) * 1e3 / launches
+        kms, nl = ctypes.c_double(), ctypes.c_int()
+        _native.check(L.nbody_hip_kernel_timer_read(ctypes.byref(kms), ctypes.byref(nl)), "timer read")
+        L.nbody_hip_kernel_timer(0)
+        plan = _native.Options()
+        _native.check(L.nbody_hip_plan(n, count, ctypes.byref(opt), ctypes.byref(plan)), "plan")
+        results.append((call_ms, kms.value / nl.value, plan))
     ms, kernel_ms, plan = min(results, key=lambda r: r[0])
     opt = _native.Options.make(variant=plan.variant, i_per_lane=plan.i_per_lane, jslice=plan.jslice,
                                compensated=plan.compensated)
