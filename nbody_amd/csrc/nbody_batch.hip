@@ -24,9 +24,13 @@ __global__ __launch_bounds__(kWave) void k_force_batch(const ForceTask* __restri
                                                        unsigned* __restrict__ counter)
 {
     __shared__ float jacc[3][kBatchBlock];
-    for (;;) {
-        int t = 0;
-        if (threadIdx.x == 0) t = (int)atomicAdd(counter, 1u);
+    // Workgroup b's first task is task b; later tickets come from the
+    // counter, offset by the grid. All workgroups start together, and their
+    // first tickets from one atomic counter would queue up behind each other
+    // (the launch measured 3 % slower that way).
+    for (bool first = true;; first = false) {
+        int t = (int)blockIdx.x;
+        if (!first && threadIdx.x == 0) t = (int)atomicAdd(counter, 1u) + (int)gridDim.x;
         // one wave per workgroup: lane 0's ticket is the wave's task, kept scalar
         t = __builtin_amdgcn_readfirstlane(t);
         if (t >= ntasks) return;

@@ -151,8 +151,9 @@ void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<cons
 // times the CU count).
 int batch_force_blocks_per_cu();
 
-// Persistent force launch: `blocks` workgroups take tasks from `counter`
-// (which must read 0 at launch) until all ntasks are done.
+// Persistent force launch: `blocks` workgroups each start with the task of
+// their own index, then take tasks from `counter` (which must read 0 at
+// launch) until all ntasks are done.
 hipError_t launch_force_batch(const ForceTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream);
 
 // Sum every body's slots, write force (x, y, z; w untouched), integrate
