@@ -265,7 +265,10 @@ float timed_force(const nbody_hip_options* opt, float* force, const float* pm, f
         set_error("cannot allocate %zu bytes of slice scratch", bytes);
         return 0.0f;
     }
-    if (hipEventRecord(d->ev0, nullptr) != hipSuccess) return 0.0f;
+    if (hipError_t e = hipEventRecord(d->ev0, nullptr); e != hipSuccess) {
+        set_error("timing failed: %s", hipGetErrorString(e));
+        return 0.0f;
+    }
     if (run_force(*d, plan, force, pm, eps2, N, 0, N, nullptr)) return 0.0f;
     float ms = 0.0f;
     hipError_t e = hipEventRecord(d->ev1, nullptr);
@@ -402,7 +405,10 @@ float integrateGravitation_HIP(float* posMass, float* velInvMass, const float* f
     std::lock_guard<std::mutex> lock(g_mutex);
     DeviceState* d;
     if (device_state(&d)) return 0.0f;
-    if (hipEventRecord(d->ev0, nullptr) != hipSuccess) return 0.0f;
+    if (hipError_t e = hipEventRecord(d->ev0, nullptr); e != hipSuccess) {
+        set_error("integrate failed: %s", hipGetErrorString(e));
+        return 0.0f;
+    }
     if (nbody_hip_integrate(posMass, velInvMass, force, dt, damping, N, nullptr)) return 0.0f;
     float ms = 0.0f;
     hipError_t e = hipEventRecord(d->ev1, nullptr);
@@ -541,6 +547,10 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
             }
         }
     }
+    if (cands.empty()) {
+        set_error("no plan of variant %d fits N = %zu", variant, N);
+        return 0.0f;
+    }
     float best = 0.0f;
     ForcePlan best_plan = cands.front();
     for (const ForcePlan& c : cands) {
@@ -552,7 +562,10 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
         float t = 0.0f;
         int batch = 1;
         for (int rep = 0; rep < 5; ++rep) {
-            if (hipEventRecord(d->ev0, nullptr) != hipSuccess) return 0.0f;
+            if (hipEventRecord(d->ev0, nullptr) != hipSuccess) {
+                set_error("autotune timing failed");
+                return 0.0f;
+            }
             for (int k = 0; k < batch; ++k)
                 if (run_force(*d, c, force, posMass, softeningSquared, N, 0, N, nullptr)) return 0.0f;
             float ms = 0.0f;
@@ -569,6 +582,10 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
             best = t;
             best_plan = c;
         }
+    }
+    if (best == 0.0f) {
+        set_error("autotune: no candidate plan could allocate its scratch for N = %zu", N);
+        return 0.0f;
     }
     d->tuned[tune_key(&base, N, N)] = best_plan;
     return best;

@@ -691,10 +691,16 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
         }
         active.swap(still);
     }
-    for (ServeReq* q : staged)
+    // Release what is still on the device once nothing runs on it. After a
+    // failed admission a request can sit in both lists, so each buffer is
+    // freed once and forgotten.
+    (void)hipStreamSynchronize(s->stream);
+    auto drop = [](ServeReq* q) {
         if (q->buf) (void)hipFree(q->buf);
-    for (ServeReq* q : active)
-        if (q->buf) (void)hipFree(q->buf);
+        q->buf = nullptr;
+    };
+    for (ServeReq* q : staged) drop(q);
+    for (ServeReq* q : active) drop(q);
     for (size_t k = 0; k < count; ++k) {
         const ServeReq& q = rq[k];
         results[k].id = q.r.id;

@@ -306,6 +306,17 @@ def test_autotune_then_default(oracle):
     assert mx <= TOL_MAX and rms <= TOL_RMS
 
 
+def test_autotune_without_a_fitting_plan():
+    """No symmetric shape fits N = 2^26 (more than 16384 blocks each): the
+    autotuner says so and launches nothing (the buffers are 16 bodies)."""
+    pm, _ = bodies(16)
+    d = dev(pm)
+    f = force_buffer(16)
+    assert _native.lib().nbody_hip_autotune(ptr(f), ptr(d), EPS2, 1 << 26, _native.SYMMETRIC) == 0.0
+    assert "no plan" in _native.last_error()
+    assert np.all(np.isnan(host(f)[:, :3]))
+
+
 @pytest.mark.slow
 def test_full_size_98304_sampled(oracle):
     """The single-system workload (N = 98304 uniform ball, eps^2 = 0.01): the
