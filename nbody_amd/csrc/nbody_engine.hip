@@ -440,6 +440,7 @@ struct ServeReq {
     int status = 0;
     float4* buf = nullptr;  // [posMass | velInvMass | force], 3 N records, then the slot planes
     bool reserved = false;  // its bytes count against the budget
+    int ics = 0;            // host initial conditions: 0 not built, 1 built, 2 failed
     std::vector<float> hpm, hvm;  // initial conditions built on the host, until uploaded
     ForceTaskBands tasks;         // its force tasks, built once it is on the device
     std::vector<BodyTask> body_tasks;
@@ -563,9 +564,14 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
         const double t = now();
         while (next < count && rq[order[next]].r.arrival_s <= t) waiting.push_back(&rq[order[next++]]);
     };
+    // A request's initial conditions, built on the host once.
+    auto build_ics = [&](ServeReq* q) {
+        if (q->ics == 0) q->ics = request_ics(q->r, q->hpm, q->hvm) ? 2 : 1;
+        return q->ics == 1;
+    };
     // Admit in arrival order while a slot is free and the budget holds: the
     // slot and the device bytes are reserved and the initial conditions are
-    // built on the host. This host work runs while a step is on the GPU.
+    // built on the host unless they already were.
     auto admit = [&] {
         while (!waiting.empty() && (int)(active.size() + staged.size()) < s->max_active) {
             ServeReq* q = waiting.front();
@@ -577,7 +583,7 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             }
             if (used + q->bytes() > s->budget) break;
             waiting.pop_front();
-            if (request_ics(q->r, q->hpm, q->hvm)) {
+            if (!build_ics(q)) {
                 q->status = 2;
                 retire(q, now());
                 continue;
@@ -645,8 +651,12 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             }
             dirty = false;
         }
-        // one step of every active request; arrivals and admissions (host
-        // initial conditions) overlap it; then wait for it
+        // One step of every active request. Arrivals and admissions overlap
+        // it, and so does building the initial conditions of the waiting
+        // requests next in line for the slots this step frees: slots free
+        // only when requests retire after the step, and the host work of
+        // admitting them (a 131072-body Plummer sphere takes ~45 ms) would
+        // otherwise leave the GPU idle between steps.
         hipError_t e = launch_force_batch(s->ftasks, (int)host_ft.size(), s->counter, s->blocks, s->stream);
         if (e == hipSuccess)
             e = launch_reduce_integrate_batch(s->btasks, (int)host_bt.size(), s->dt, s->damping, s->counter,
@@ -654,6 +664,9 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
         if (e == hipSuccess) {
             arrivals();
             admit();
+            size_t leaving = 0;
+            for (const ServeReq* q : active) leaving += q->done + 1 >= q->r.steps ? 1 : 0;
+            for (size_t k = 0; k < waiting.size() && k < leaving; ++k) (void)build_ics(waiting[k]);
             e = hipStreamSynchronize(s->stream);
         }
         if (e != hipSuccess) {
