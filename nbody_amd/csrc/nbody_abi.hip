@@ -1,7 +1,7 @@
 // extern "C" entry points of libnbody_hip.so (declared in include/nbody_hip.h).
 //
 // Error convention of the reference (src/chError.h:86-91 + src/nbody.cu:385-386):
-// a failing table entry returns 0.0f and the driver skips it. Here the reason is
+// a failing table entry returns 0.0f and the reference driver skips it. Here the reason is
 // also kept per thread for nbody_hip_last_error().
 #include "nbody_internal.h"
 
