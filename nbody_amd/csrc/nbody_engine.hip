@@ -30,9 +30,10 @@ hipError_t dev_alloc(T** p, size_t count)
     return hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T));
 }
 
-// Copy a task list to a device buffer that grows (to twice the need) when short.
+// Copy a task list to a device buffer that grows (to twice the need) when
+// short, on `stream`; the host vector must stay unchanged until it is done.
 template <class T>
-int upload(T** dev, size_t* cap, const std::vector<T>& host)
+int upload(T** dev, size_t* cap, const std::vector<T>& host, hipStream_t stream)
 {
     if (host.size() > *cap) {
         if (*dev) (void)hipFree(*dev);
@@ -45,7 +46,8 @@ int upload(T** dev, size_t* cap, const std::vector<T>& host)
         *cap = 2 * host.size();
     }
     if (host.empty()) return OK;
-    if (hipError_t e = hipMemcpy(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice); e != hipSuccess) {
+    if (hipError_t e = hipMemcpyAsync(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream);
+        e != hipSuccess) {
         set_error("task upload: %s", hipGetErrorString(e));
         return E_HIP;
     }
@@ -465,21 +467,26 @@ struct nbody_hip_server {
     int max_active = 1;
     float dt = 0.f, damping = 1.f;
     bool keep = false;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the steps
+    hipStream_t up = nullptr;      // uploads during a step (non-blocking: it does not wait for the step)
     unsigned* counter = nullptr;
-    ForceTask* ftasks = nullptr;
-    BodyTask* btasks = nullptr;
-    size_t ft_cap = 0, bt_cap = 0;
+    // two task lists: a step reads one while the next step's is uploaded into the other
+    ForceTask* ftasks[2] = {nullptr, nullptr};
+    BodyTask* btasks[2] = {nullptr, nullptr};
+    size_t ft_cap[2] = {0, 0}, bt_cap[2] = {0, 0};
     int blocks = 0;
     int peak_active = 0;
     std::map<uint64_t, std::pair<std::vector<float>, std::vector<float>>> states;
 
     void release()
     {
-        if (ftasks) (void)hipFree(ftasks);
-        if (btasks) (void)hipFree(btasks);
+        for (int k = 0; k < 2; ++k) {
+            if (ftasks[k]) (void)hipFree(ftasks[k]);
+            if (btasks[k]) (void)hipFree(btasks[k]);
+        }
         if (counter) (void)hipFree(counter);
         if (stream) (void)hipStreamDestroy(stream);
+        if (up) (void)hipStreamDestroy(up);
     }
 };
 
@@ -503,6 +510,7 @@ nbody_hip_server* nbody_hip_server_create(size_t budget_bytes, int max_active, f
     s->keep = keep_states != 0;
     s->blocks = d->cus * batch_force_blocks_per_cu();
     hipError_t e = hipStreamCreate(&s->stream);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
     if (e == hipSuccess) e = dev_alloc(&s->counter, 1);
     if (e == hipSuccess) e = hipMemset(s->counter, 0, sizeof(unsigned));
     if (e != hipSuccess) {
@@ -540,13 +548,15 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     const auto t0 = std::chrono::steady_clock::now();
     auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
     std::deque<ServeReq*> waiting;
-    std::vector<ServeReq*> staged;  // admitted, initial conditions built on the host, not yet on the device
-    std::vector<ServeReq*> active;
+    std::vector<ServeReq*> staged;   // admitted, initial conditions on the host, not yet on the device
+    std::vector<ServeReq*> joining;  // on the device (uploads queued on s->up), in the next step's list
+    std::vector<ServeReq*> active;   // in the current task list
     std::vector<ForceTask> host_ft;
     std::vector<BodyTask> host_bt;
-    size_t next = 0, finished = 0, used = 0;
+    int cur = 0;                     // the task list that describes `active`
+    int nft[2] = {0, 0}, nbt[2] = {0, 0};
+    size_t next = 0, finished = 0, used = 0, freed = 0;
     double work = 0.0;
-    bool dirty = true;
     int rc = OK;
     auto retire = [&](ServeReq* q, double t) {
         q->finish = t;
@@ -571,9 +581,12 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     };
     // Admit in arrival order while a slot is free and the budget holds: the
     // slot and the device bytes are reserved and the initial conditions are
-    // built on the host unless they already were.
+    // built on the host unless they already were. A request's slot is free
+    // once its last step is launched (`freed` counts those of the running
+    // step); its bytes count until it retires after that step.
     auto admit = [&] {
-        while (!waiting.empty() && (int)(active.size() + staged.size()) < s->max_active) {
+        while (!waiting.empty() &&
+               active.size() - freed + joining.size() + staged.size() < (size_t)s->max_active) {
             ServeReq* q = waiting.front();
             if (q->bytes() > s->budget) {
                 q->status = 1;
@@ -598,91 +611,114 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             staged.push_back(q);
         }
     };
-    while (finished < count && rc == OK) {
-        arrivals();
-        admit();
-        // staged requests onto the device while it is idle between steps
+    // Staged requests onto the device: allocate, queue the uploads of their
+    // initial conditions on s->up, build their task bands. They join the
+    // next task list.
+    auto stage = [&] {
         for (ServeReq* q : staged) {
             hipError_t e = hipMalloc(reinterpret_cast<void**>(&q->buf), q->bytes());
+            if (e != hipSuccess) q->buf = nullptr;
             if (e == hipSuccess)
-                e = hipMemcpy(q->buf, q->hpm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
+                e = hipMemcpyAsync(q->buf, q->hpm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice, s->up);
             if (e == hipSuccess)
-                e = hipMemcpy(q->buf + q->r.N, q->hvm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice);
+                e = hipMemcpyAsync(q->buf + q->r.N, q->hvm.data(), q->r.N * sizeof(float4), hipMemcpyHostToDevice,
+                                   s->up);
+            joining.push_back(q);  // from here its buffer is released with the others on any error
             if (e != hipSuccess) {
                 set_error("admitting request %llu failed: %s", (unsigned long long)q->r.id, hipGetErrorString(e));
-                rc = E_HIP;
-                break;
+                return E_HIP;
             }
-            std::vector<float>().swap(q->hpm);
-            std::vector<float>().swap(q->hvm);
             build_force_task_bands(q->tasks, q->buf, q->part(), q->r.N, q->r.softeningSquared);
             q->body_tasks.clear();
             append_body_tasks(q->body_tasks, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->part(), q->r.N);
-            active.push_back(q);
-            dirty = true;
         }
-        if (rc) break;
         staged.clear();
-        s->peak_active = std::max(s->peak_active, (int)active.size());
+        return OK;
+    };
+    // The task lists of `set`, concatenated band by band, queued on s->up into list k.
+    auto build_list = [&](const std::vector<ServeReq*>& set, int k) {
+        std::vector<const ForceTaskBands*> bands;
+        bands.reserve(set.size());
+        host_bt.clear();
+        for (ServeReq* q : set) {
+            bands.push_back(&q->tasks);
+            host_bt.insert(host_bt.end(), q->body_tasks.begin(), q->body_tasks.end());
+        }
+        concat_force_task_bands(host_ft, bands);
+        nft[k] = (int)host_ft.size();
+        nbt[k] = (int)host_bt.size();
+        if (int r = upload(&s->ftasks[k], &s->ft_cap[k], host_ft, s->up)) return r;
+        return upload(&s->btasks[k], &s->bt_cap[k], host_bt, s->up);
+    };
+    // The uploads are done: the host copies of the initial conditions can go.
+    auto uploaded = [&]() -> int {
+        HIP_TRY(hipStreamSynchronize(s->up));
+        for (ServeReq* q : joining) {
+            std::vector<float>().swap(q->hpm);
+            std::vector<float>().swap(q->hvm);
+        }
+        return OK;
+    };
+    while (finished < count && rc == OK) {
+        arrivals();
+        admit();
         if (active.empty()) {
-            if (next < count) {
-                const double wait = rq[order[next]].r.arrival_s - now();
-                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            // nothing on the device: what was admitted goes up before a step
+            if (staged.empty()) {
+                if (next < count) {
+                    const double wait = rq[order[next]].r.arrival_s - now();
+                    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+                }
+                continue;
             }
-            continue;
+            if ((rc = stage()) || (rc = build_list(joining, cur)) || (rc = uploaded())) break;
+            active.swap(joining);
+            joining.clear();
         }
-        if (dirty) {
-            // the resident requests' prebuilt task lists, concatenated band by band
-            std::vector<const ForceTaskBands*> bands;
-            bands.reserve(active.size());
-            host_bt.clear();
-            for (ServeReq* q : active) {
-                bands.push_back(&q->tasks);
-                host_bt.insert(host_bt.end(), q->body_tasks.begin(), q->body_tasks.end());
-            }
-            concat_force_task_bands(host_ft, bands);
-            if (int r = upload(&s->ftasks, &s->ft_cap, host_ft); r) {
-                rc = r;
-                break;
-            }
-            if (int r = upload(&s->btasks, &s->bt_cap, host_bt); r) {
-                rc = r;
-                break;
-            }
-            dirty = false;
-        }
-        // One step of every active request. Arrivals and admissions overlap
-        // it, and so does building the initial conditions of the waiting
-        // requests next in line for the slots this step frees: slots free
-        // only when requests retire after the step, and the host work of
-        // admitting them (a 131072-body Plummer sphere takes ~45 ms) would
-        // otherwise leave the GPU idle between steps.
-        hipError_t e = launch_force_batch(s->ftasks, (int)host_ft.size(), s->counter, s->blocks, s->stream);
+        s->peak_active = std::max(s->peak_active, (int)active.size());
+        // One step of every active request from list `cur`. While it runs
+        // the host takes arrivals and admissions, builds the initial
+        // conditions of the waiting requests next in line for the slots this
+        // step frees (slots free only after the step, and a 131072-body
+        // Plummer sphere takes ~45 ms on the host), uploads the admitted
+        // requests, and builds and uploads the next step's task list into
+        // the other list. Only retiring the finished requests waits for the
+        // step.
+        hipError_t e = launch_force_batch(s->ftasks[cur], nft[cur], s->counter, s->blocks, s->stream);
         if (e == hipSuccess)
-            e = launch_reduce_integrate_batch(s->btasks, (int)host_bt.size(), s->dt, s->damping, s->counter,
-                                              s->stream);
-        if (e == hipSuccess) {
-            arrivals();
-            admit();
-            size_t leaving = 0;
-            for (const ServeReq* q : active) leaving += q->done + 1 >= q->r.steps ? 1 : 0;
-            for (size_t k = 0; k < waiting.size() && k < leaving; ++k) (void)build_ics(waiting[k]);
-            e = hipStreamSynchronize(s->stream);
-        }
+            e = launch_reduce_integrate_batch(s->btasks[cur], nbt[cur], s->dt, s->damping, s->counter, s->stream);
         if (e != hipSuccess) {
             set_error("server step: %s", hipGetErrorString(e));
             rc = E_HIP;
             break;
         }
+        std::vector<ServeReq*> next_active;
+        next_active.reserve(active.size() + s->max_active);
+        for (ServeReq* q : active)
+            if (q->done + 1 < q->r.steps) next_active.push_back(q);
+        freed = active.size() - next_active.size();
+        arrivals();
+        admit();
+        for (size_t k = 0; k < waiting.size() && k < freed; ++k) (void)build_ics(waiting[k]);
+        bool flip = false;
+        if ((rc = stage())) break;
+        if (freed || !joining.empty()) {
+            next_active.insert(next_active.end(), joining.begin(), joining.end());
+            if (!next_active.empty() && (rc = build_list(next_active, 1 - cur))) break;
+            flip = true;
+        }
+        e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            set_error("server step: %s", hipGetErrorString(e));
+            rc = E_HIP;
+            break;
+        }
+        if ((rc = uploaded())) break;
+        freed = 0;
         const double t = now();
-        std::vector<ServeReq*> still;
-        still.reserve(active.size());
         for (ServeReq* q : active) {
             work += (double)q->r.N * (double)q->r.N;
-            if (++q->done < q->r.steps) {
-                still.push_back(q);
-                continue;
-            }
+            if (++q->done < q->r.steps) continue;
             if (s->keep) {
                 auto& st = s->states[q->r.id];
                 st.first.resize(4 * q->r.N);
@@ -700,19 +736,22 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
             std::vector<ForceTask>().swap(q->tasks.diag);
             std::vector<BodyTask>().swap(q->body_tasks);
             retire(q, t);
-            dirty = true;
         }
-        active.swap(still);
+        active.swap(next_active);
+        joining.clear();
+        if (flip) cur = 1 - cur;
     }
     // Release what is still on the device once nothing runs on it. After a
-    // failed admission a request can sit in both lists, so each buffer is
+    // failure a request can sit in more than one list, so each buffer is
     // freed once and forgotten.
     (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->up);
     auto drop = [](ServeReq* q) {
         if (q->buf) (void)hipFree(q->buf);
         q->buf = nullptr;
     };
     for (ServeReq* q : staged) drop(q);
+    for (ServeReq* q : joining) drop(q);
     for (ServeReq* q : active) drop(q);
     for (size_t k = 0; k < count; ++k) {
         const ServeReq& q = rq[k];
