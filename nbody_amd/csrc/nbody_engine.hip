@@ -661,9 +661,10 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     };
     while (finished < count && rc == OK) {
         arrivals();
-        admit();
         if (active.empty()) {
-            // nothing on the device: what was admitted goes up before a step
+            // nothing on the device: admit and upload before a step (with
+            // requests resident, admission happens while their step runs)
+            admit();
             if (staged.empty()) {
                 if (next < count) {
                     const double wait = rq[order[next]].r.arrival_s - now();
