@@ -133,13 +133,41 @@ __global__ __launch_bounds__(3 * kBodyTaskBodies) void k_reduce_integrate_batch(
     task.vm[i] = v;
 }
 
-// Work of a task in VALU instructions per lane (16 + rsq per pair step,
-// 12 + rsq per one-sided interaction), for ordering only.
-double task_cost(const ForceTask& t)
+// Work of a ROW_COMP task in VALU instructions per lane (12 + rsq per
+// one-sided interaction), for ordering those tasks.
+double row_cost(const ForceTask& t) { return (double)t.len * 13.0; }
+
+// Cost of a block task in halves of a 64 x 64 tile block: a pair meets the
+// tiles of block Q that hold bodies (one-wave pairs skip the padding tiles)
+// with all 8 tiles of block P, 16 per Q tile, 128 for a full pair; a
+// diagonal block of T tiles takes each pair once, T^2, 64 when full.
+constexpr int kMaxTileKey = 16 * (kBatchBlock / kWave);
+
+int tile_key(const ForceTask& t)
 {
-    if (t.kind == TASK_PAIR) return (double)kBatchBlock * kBatchBlock / kWave * 17.0;
-    if (t.kind == TASK_DIAG) return (double)t.len * t.len / 2 / kWave * 17.0;
-    return (double)t.len * 13.0;
+    if (t.kind == TASK_PAIR) {
+        const int qlen = std::min(kBatchBlock, t.n - t.b * kBatchBlock);
+        return 16 * ((qlen + kWave - 1) / kWave);
+    }
+    const int tiles = (t.len + kWave - 1) / kWave;
+    return tiles * tiles;
+}
+
+// Append block tasks to `out` most expensive first: a stable counting sort
+// on tile_key, linear in the task count. `each(f)` calls f on every task.
+template <class Each>
+void append_by_tile_key(std::vector<ForceTask>& out, const Each& each)
+{
+    size_t start[kMaxTileKey + 1] = {};
+    each([&](const ForceTask& t) { ++start[tile_key(t)]; });
+    size_t pos = out.size();
+    for (int k = kMaxTileKey; k >= 0; --k) {
+        const size_t c = start[k];
+        start[k] = pos;
+        pos += c;
+    }
+    out.resize(pos);
+    each([&](const ForceTask& t) { out[start[tile_key(t)]++] = t; });
 }
 
 }  // namespace
@@ -222,24 +250,17 @@ void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float
 
 void order_force_tasks(std::vector<ForceTask>& tasks)
 {
-    // The costs fall in three bands: a ROW_COMP task (len > 2^18) outweighs a
-    // PAIR task, which outweighs any DIAG task (at most 512 bodies, each pair
-    // once). A stable sort within the two outer bands and the pairs as they
-    // come give the same order as a full stable sort by cost, in linear time
-    // for the many pairs.
-    std::vector<ForceTask> big, pairs, diag;
-    for (const ForceTask& t : tasks) {
-        if (t.kind == TASK_PAIR) pairs.push_back(t);
-        else if (t.kind == TASK_ROW_COMP) big.push_back(t);
-        else diag.push_back(t);
-    }
-    const auto by_cost = [](const ForceTask& a, const ForceTask& b) { return task_cost(a) > task_cost(b); };
-    std::stable_sort(big.begin(), big.end(), by_cost);
-    std::stable_sort(diag.begin(), diag.end(), by_cost);
-    tasks.clear();
-    tasks.insert(tasks.end(), big.begin(), big.end());
-    tasks.insert(tasks.end(), pairs.begin(), pairs.end());
-    tasks.insert(tasks.end(), diag.begin(), diag.end());
+    // A ROW_COMP task (64 bodies against more than 2^18) outweighs any block
+    // task: those go first, by cost; the block pairs and diagonal blocks
+    // follow by tile_key, so the cheapest tasks close the launch.
+    std::vector<ForceTask> big, blocks;
+    for (const ForceTask& t : tasks) (t.kind == TASK_ROW_COMP ? big : blocks).push_back(t);
+    std::stable_sort(big.begin(), big.end(),
+                     [](const ForceTask& a, const ForceTask& b) { return row_cost(a) > row_cost(b); });
+    tasks.assign(big.begin(), big.end());
+    append_by_tile_key(tasks, [&](const auto& f) {
+        for (const ForceTask& t : blocks) f(t);
+    });
 }
 
 void build_force_task_bands(ForceTaskBands& out, const float4* pm, float* part, size_t n, float eps2)
@@ -272,8 +293,13 @@ void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<cons
     std::stable_sort(by_size.begin(), by_size.end(),
                      [](const ForceTaskBands* a, const ForceTaskBands* b) { return a->big[0].len > b->big[0].len; });
     for (const ForceTaskBands* b : by_size) out.insert(out.end(), b->big.begin(), b->big.end());
-    for (const ForceTaskBands* b : systems) out.insert(out.end(), b->pairs.begin(), b->pairs.end());
-    for (const ForceTaskBands* b : systems) out.insert(out.end(), b->diag.begin(), b->diag.end());
+    // then every system's block tasks, most expensive first
+    append_by_tile_key(out, [&](const auto& f) {
+        for (const ForceTaskBands* b : systems)
+            for (const ForceTask& t : b->pairs) f(t);
+        for (const ForceTaskBands* b : systems)
+            for (const ForceTask& t : b->diag) f(t);
+    });
 }
 
 int batch_force_blocks_per_cu()

@@ -134,14 +134,17 @@ void append_force_tasks(std::vector<ForceTask>& out, const float4* pm, float* pa
 void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float4* force, const float* part,
                        size_t n);
 // Most expensive tasks first (stable), so the end of the launch holds the
-// cheapest ones.
+// cheapest ones: compensated rows by cost, then the block tasks by their
+// tiles (a block pair whose second block is a system's ragged last block
+// skips its padding tiles; a diagonal block is about half a pair), the
+// latter by a counting sort, linear in the task count.
 void order_force_tasks(std::vector<ForceTask>& tasks);
 
-// One system's force tasks split by cost band: compensated rows of a large
-// system (each outweighs a pair), block pairs, diagonal blocks (each about
-// half a pair). Concatenating the bands of many systems band by band gives a
-// most-expensive-first order without sorting, which is what a server that
-// rebuilds its task list whenever a request joins or leaves needs.
+// One system's force tasks split by kind: compensated rows of a large system
+// (each outweighs a block task), block pairs, diagonal blocks. A server that
+// rebuilds its task list whenever a request joins or leaves keeps these per
+// request and concatenates them in the same order as order_force_tasks, in
+// time linear in the task count.
 struct ForceTaskBands {
     std::vector<ForceTask> big, pairs, diag;
 };
