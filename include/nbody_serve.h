@@ -6,11 +6,12 @@
  * The server keeps every admitted request's positions and velocities resident
  * in HBM and runs continuous batching: each iteration advances every active
  * request by one step with one persistent force launch over all of them (the
- * batched work queue of nbody_engine.h) and one reduce + integrate launch; between
- * iterations finished requests retire and waiting ones are admitted, in
- * arrival order, while a slot is free and their device memory fits the HBM
- * budget. A request's result depends only on the request, never on what else
- * is in the batch.
+ * batched work queue of nbody_engine.h) and one reduce + integrate launch.
+ * Waiting requests are admitted in arrival order while a slot is free and
+ * their device memory fits the HBM budget; while a step runs they are
+ * uploaded and join the next step. A request's slot frees when its last step
+ * is launched, its memory when that step has finished. A request's result
+ * depends only on the request, never on what else is in the batch.
  *
  * Functions return 0 or a nonzero code (nbody_hip_last_error()).
  */
