@@ -128,10 +128,20 @@ struct nbody_hip_ensemble {
     float4* vm = nullptr;
     float4* force = nullptr;
     float* part = nullptr;         // every system's slot planes, back to back
-    ForceTask* ftasks = nullptr;
-    BodyTask* btasks = nullptr;
-    unsigned* counter = nullptr;
-    int nft = 0, nbt = 0;
+    // The systems in kGroups groups of about equal work, each with its own
+    // task lists and work-queue counter; a batched step forks, runs each
+    // group's force and slot-sum launches on its own stream, and joins, so
+    // one group's launches fill another's drain (two groups measured best:
+    // four were no faster than one, eight slower).
+    static constexpr int kGroups = 2;
+    int ngroups = 1;
+    ForceTask* ftasks[kGroups] = {};
+    BodyTask* btasks[kGroups] = {};
+    int nft[kGroups] = {}, nbt[kGroups] = {};
+    unsigned* counter = nullptr;   // kGroups counters, kCounterStride words apart
+    static constexpr int kCounterStride = 64;
+    hipStream_t side[kGroups] = {};    // group g > 0 runs on side[g]
+    hipEvent_t fork = nullptr, join[kGroups] = {};
     int blocks = 0;
     std::vector<ForcePlan> plans;  // per-system mode
     Buffer scratch;
@@ -143,17 +153,38 @@ struct nbody_hip_ensemble {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* p : {(void*)pm, (void*)vm, (void*)force, (void*)part, (void*)ftasks, (void*)btasks, (void*)counter,
-                        scratch.ptr})
+        for (void* p : {(void*)pm, (void*)vm, (void*)force, (void*)part, (void*)counter, scratch.ptr})
             if (p) (void)hipFree(p);
+        for (int g = 0; g < kGroups; ++g) {
+            if (ftasks[g]) (void)hipFree(ftasks[g]);
+            if (btasks[g]) (void)hipFree(btasks[g]);
+            if (side[g]) (void)hipStreamDestroy(side[g]);
+            if (join[g]) (void)hipEventDestroy(join[g]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
+        if (fork) (void)hipEventDestroy(fork);
         timing.destroy();
+    }
+
+    int queue_group(int g, hipStream_t s)
+    {
+        unsigned* c = counter + g * kCounterStride;
+        HIP_TRY(launch_force_batch(ftasks[g], nft[g], c, blocks, s));
+        HIP_TRY(launch_reduce_integrate_batch(btasks[g], nbt[g], dt, damping, c, s));
+        return OK;
     }
 
     int queue_batched()
     {
-        HIP_TRY(launch_force_batch(ftasks, nft, counter, blocks, stream));
-        HIP_TRY(launch_reduce_integrate_batch(btasks, nbt, dt, damping, counter, stream));
+        if (ngroups == 1) return queue_group(0, stream);
+        HIP_TRY(hipEventRecord(fork, stream));
+        for (int g = 1; g < ngroups; ++g) HIP_TRY(hipStreamWaitEvent(side[g], fork, 0));
+        if (int r = queue_group(0, stream)) return r;
+        for (int g = 1; g < ngroups; ++g) {
+            if (int r = queue_group(g, side[g])) return r;
+            HIP_TRY(hipEventRecord(join[g], side[g]));
+        }
+        for (int g = 1; g < ngroups; ++g) HIP_TRY(hipStreamWaitEvent(stream, join[g], 0));
         return OK;
     }
 
@@ -338,30 +369,55 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
     if (err == hipSuccess) err = dev_alloc(&e->vm, total);
     if (err == hipSuccess) err = dev_alloc(&e->force, total);
     if (err == hipSuccess) err = dev_alloc(&e->part, part_total);
-    if (err == hipSuccess) err = dev_alloc(&e->counter, 1);
+    if (err == hipSuccess) err = dev_alloc(&e->counter, (size_t)nbody_hip_ensemble::kGroups * nbody_hip_ensemble::kCounterStride);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+    for (int g = 1; g < nbody_hip_ensemble::kGroups && err == hipSuccess; ++g) {
+        err = hipStreamCreate(&e->side[g]);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&e->join[g], hipEventDisableTiming);
+    }
     if (err == hipSuccess) err = e->scratch.reserve(scratch);
     if (err == hipSuccess) err = hipMemcpy(e->pm, posMass, total * sizeof(float4), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemcpy(e->vm, velInvMass, total * sizeof(float4), hipMemcpyHostToDevice);
     if (err == hipSuccess) err = hipMemset(e->force, 0, total * sizeof(float4));
     if (err == hipSuccess) {
-        std::vector<ForceTask> ft;
-        std::vector<BodyTask> bt;
-        for (size_t s = 0; s < count; ++s) {
-            append_force_tasks(ft, e->pm + e->off[s], e->part + part_off[s], N[s], softeningSquared[s]);
-            append_body_tasks(bt, e->pm + e->off[s], e->vm + e->off[s], e->force + e->off[s], e->part + part_off[s],
-                              N[s]);
+        // two groups of about equal work (sum of N^2), the largest systems
+        // dealt first to the lighter group
+        std::vector<size_t> by_work(count);
+        std::iota(by_work.begin(), by_work.end(), 0);
+        std::stable_sort(by_work.begin(), by_work.end(), [&](size_t a, size_t b) { return N[a] > N[b]; });
+        std::vector<int> group(count, 0);
+        double load[nbody_hip_ensemble::kGroups] = {};
+        for (size_t s : by_work) {
+            int g = 0;
+            for (int k = 1; k < nbody_hip_ensemble::kGroups; ++k)
+                if (load[k] < load[g]) g = k;
+            group[s] = g;
+            load[g] += (double)N[s] * (double)N[s];
         }
-        order_force_tasks(ft);
-        e->nft = (int)ft.size();
-        e->nbt = (int)bt.size();
-        err = dev_alloc(&e->ftasks, ft.size());
-        if (err == hipSuccess) err = dev_alloc(&e->btasks, bt.size());
-        if (err == hipSuccess)
-            err = hipMemcpy(e->ftasks, ft.data(), ft.size() * sizeof(ForceTask), hipMemcpyHostToDevice);
-        if (err == hipSuccess)
-            err = hipMemcpy(e->btasks, bt.data(), bt.size() * sizeof(BodyTask), hipMemcpyHostToDevice);
+        e->ngroups = (int)std::min<size_t>(count, nbody_hip_ensemble::kGroups);
+        for (int g = 0; g < e->ngroups && err == hipSuccess; ++g) {
+            std::vector<ForceTask> ft;
+            std::vector<BodyTask> bt;
+            for (size_t s = 0; s < count; ++s) {
+                if (group[s] != g) continue;
+                append_force_tasks(ft, e->pm + e->off[s], e->part + part_off[s], N[s], softeningSquared[s]);
+                append_body_tasks(bt, e->pm + e->off[s], e->vm + e->off[s], e->force + e->off[s],
+                                  e->part + part_off[s], N[s]);
+            }
+            order_force_tasks(ft);
+            e->nft[g] = (int)ft.size();
+            e->nbt[g] = (int)bt.size();
+            err = dev_alloc(&e->ftasks[g], ft.size());
+            if (err == hipSuccess) err = dev_alloc(&e->btasks[g], bt.size());
+            if (err == hipSuccess)
+                err = hipMemcpy(e->ftasks[g], ft.data(), ft.size() * sizeof(ForceTask), hipMemcpyHostToDevice);
+            if (err == hipSuccess)
+                err = hipMemcpy(e->btasks[g], bt.data(), bt.size() * sizeof(BodyTask), hipMemcpyHostToDevice);
+        }
     }
-    if (err == hipSuccess) err = hipMemset(e->counter, 0, sizeof(unsigned));
+    if (err == hipSuccess)
+        err = hipMemset(e->counter, 0,
+                        (size_t)nbody_hip_ensemble::kGroups * nbody_hip_ensemble::kCounterStride * sizeof(unsigned));
     if (err == hipSuccess) err = e->timing.init();
     if (err != hipSuccess) {
         set_error("ensemble create failed: %s", hipGetErrorString(err));
@@ -425,6 +481,8 @@ void nbody_hip_ensemble_destroy(nbody_hip_ensemble* e)
     std::lock_guard<std::mutex> lock(g_mutex);
     if (!e) return;
     (void)hipStreamSynchronize(e->stream);
+    for (int g = 1; g < nbody_hip_ensemble::kGroups; ++g)
+        if (e->side[g]) (void)hipStreamSynchronize(e->side[g]);
     e->release();
     delete e;
 }
