@@ -21,8 +21,8 @@ diagnostic), and a CPU baseline: the reference's CPU_SOA_tiled
 build is present, else the oracle's restatement of it, timed on this host's
 cores on a bounded number of the same steps.
 
-Other workloads (--workload): ensemble (512 systems, one persistent launch
-per step), served (a Poisson request stream), plummer (N = 2^20), share (one
+Other workloads (--workload): ensemble (512 systems, batched persistent
+launches per step), served (a Poisson request stream), plummer (N = 2^20), share (one
 GPU's i-range share of the N = 98304 system split over --shares GPUs).
 """
 import argparse
@@ -167,8 +167,9 @@ def traffic_from_profiles(workload_key):
 
 def bench_ensemble(args):
     """512 independent uniform balls, N log-uniform in [256, 16384] (seeded),
-    eps^2 = 0.01, dt = 0.016, 50 steps: one persistent force launch + one
-    integrate launch per step replayed from a graph, against every system
+    eps^2 = 0.01, dt = 0.016, 50 steps: batched steps replayed from a graph
+    (the systems in two groups, each a persistent force launch and a
+    slot-sum-and-integrate launch on its own stream), against every system
     run on its own with its best single-system plan."""
     import numpy as np
     from nbody_amd import _native, ics
