@@ -49,8 +49,9 @@ void nbody_hip_system_destroy(nbody_hip_system *s);
 typedef struct nbody_hip_ensemble nbody_hip_ensemble;
 
 enum {
-    NBODY_HIP_BATCHED_GRAPH = 0, /* one persistent force launch + one integrate launch per step, from a graph */
-    NBODY_HIP_BATCHED = 1,       /* the same two launches per step, without a graph */
+    NBODY_HIP_BATCHED_GRAPH = 0, /* per step, for each of two groups of systems on its own stream: one
+                                    persistent force launch + one slot-sum-and-integrate launch; from a graph */
+    NBODY_HIP_BATCHED = 1,       /* the same launches per step, without a graph */
     NBODY_HIP_PER_SYSTEM = 2     /* every system on its own: its best single-system plan, one after another */
 };
 
