@@ -501,6 +501,7 @@ struct ServeReq {
     float4* buf = nullptr;  // [posMass | velInvMass | force], 3 N records, then the slot planes
     bool reserved = false;  // its bytes count against the budget
     int ics = 0;            // host initial conditions: 0 not built, 1 built, 2 failed
+    int group = 0;          // which of the server's two launch groups steps it
     std::vector<float> hpm, hvm;  // initial conditions built on the host, until uploaded
     ForceTaskBands tasks;         // its force tasks, built once it is on the device
     std::vector<BodyTask> body_tasks;
@@ -525,26 +526,36 @@ struct nbody_hip_server {
     int max_active = 1;
     float dt = 0.f, damping = 1.f;
     bool keep = false;
-    hipStream_t stream = nullptr;  // the steps
+    hipStream_t stream = nullptr;  // the steps (group 0)
+    hipStream_t side = nullptr;    // group 1's launches
     hipStream_t up = nullptr;      // uploads during a step (non-blocking: it does not wait for the step)
-    unsigned* counter = nullptr;
-    // two task lists: a step reads one while the next step's is uploaded into the other
-    ForceTask* ftasks[2] = {nullptr, nullptr};
-    BodyTask* btasks[2] = {nullptr, nullptr};
-    size_t ft_cap[2] = {0, 0}, bt_cap[2] = {0, 0};
+    hipEvent_t fork = nullptr, join = nullptr;
+    static constexpr int kCounterStride = 64;
+    unsigned* counter = nullptr;   // one per group, kCounterStride words apart
+    // The resident requests in two groups of about equal work, each with its
+    // own task lists and counter, launched on two streams so one group's
+    // launches fill the other's drain. Two task lists per group: a step
+    // reads one while the next step's is uploaded into the other.
+    ForceTask* ftasks[2][2] = {};
+    BodyTask* btasks[2][2] = {};
+    size_t ft_cap[2][2] = {}, bt_cap[2][2] = {};
     int blocks = 0;
     int peak_active = 0;
     std::map<uint64_t, std::pair<std::vector<float>, std::vector<float>>> states;
 
     void release()
     {
-        for (int k = 0; k < 2; ++k) {
-            if (ftasks[k]) (void)hipFree(ftasks[k]);
-            if (btasks[k]) (void)hipFree(btasks[k]);
-        }
+        for (int k = 0; k < 2; ++k)
+            for (int g = 0; g < 2; ++g) {
+                if (ftasks[k][g]) (void)hipFree(ftasks[k][g]);
+                if (btasks[k][g]) (void)hipFree(btasks[k][g]);
+            }
         if (counter) (void)hipFree(counter);
         if (stream) (void)hipStreamDestroy(stream);
+        if (side) (void)hipStreamDestroy(side);
         if (up) (void)hipStreamDestroy(up);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
     }
 };
 
@@ -568,9 +579,12 @@ nbody_hip_server* nbody_hip_server_create(size_t budget_bytes, int max_active, f
     s->keep = keep_states != 0;
     s->blocks = d->cus * batch_force_blocks_per_cu();
     hipError_t e = hipStreamCreate(&s->stream);
+    if (e == hipSuccess) e = hipStreamCreate(&s->side);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
-    if (e == hipSuccess) e = dev_alloc(&s->counter, 1);
-    if (e == hipSuccess) e = hipMemset(s->counter, 0, sizeof(unsigned));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = dev_alloc(&s->counter, 2 * nbody_hip_server::kCounterStride);
+    if (e == hipSuccess) e = hipMemset(s->counter, 0, 2 * nbody_hip_server::kCounterStride * sizeof(unsigned));
     if (e != hipSuccess) {
         set_error("server create failed: %s", hipGetErrorString(e));
         s->release();
@@ -609,16 +623,18 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     std::vector<ServeReq*> staged;   // admitted, initial conditions on the host, not yet on the device
     std::vector<ServeReq*> joining;  // on the device (uploads queued on s->up), in the next step's list
     std::vector<ServeReq*> active;   // in the current task list
-    std::vector<ForceTask> host_ft;
-    std::vector<BodyTask> host_bt;
-    int cur = 0;                     // the task list that describes `active`
-    int nft[2] = {0, 0}, nbt[2] = {0, 0};
+    std::vector<ForceTask> host_ft[2];
+    std::vector<BodyTask> host_bt[2];
+    int cur = 0;                     // the task lists that describe `active`
+    int nft[2][2] = {}, nbt[2][2] = {};  // [list][group]
+    double load[2] = {0.0, 0.0};     // resident work (sum of N^2) per group
     size_t next = 0, finished = 0, used = 0, freed = 0;
     double work = 0.0;
     int rc = OK;
     auto retire = [&](ServeReq* q, double t) {
         q->finish = t;
         if (q->buf) {
+            load[q->group] -= (double)q->r.N * (double)q->r.N;
             (void)hipFree(q->buf);
             q->buf = nullptr;
         }
@@ -687,26 +703,54 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
                 return E_HIP;
             }
             build_force_task_bands(q->tasks, q->buf, q->part(), q->r.N, q->r.softeningSquared);
+            q->group = load[1] < load[0] ? 1 : 0;
+            load[q->group] += (double)q->r.N * (double)q->r.N;
             q->body_tasks.clear();
             append_body_tasks(q->body_tasks, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->part(), q->r.N);
         }
         staged.clear();
         return OK;
     };
-    // The task lists of `set`, concatenated band by band, queued on s->up into list k.
-    auto build_list = [&](const std::vector<ServeReq*>& set, int k) {
-        std::vector<const ForceTaskBands*> bands;
-        bands.reserve(set.size());
-        host_bt.clear();
-        for (ServeReq* q : set) {
-            bands.push_back(&q->tasks);
-            host_bt.insert(host_bt.end(), q->body_tasks.begin(), q->body_tasks.end());
+    // The task lists of `set`, per group, queued on s->up into list k.
+    auto build_list = [&](const std::vector<ServeReq*>& set, int k) -> int {
+        for (int g = 0; g < 2; ++g) {
+            std::vector<const ForceTaskBands*> bands;
+            bands.reserve(set.size());
+            host_bt[g].clear();
+            for (ServeReq* q : set) {
+                if (q->group != g) continue;
+                bands.push_back(&q->tasks);
+                host_bt[g].insert(host_bt[g].end(), q->body_tasks.begin(), q->body_tasks.end());
+            }
+            concat_force_task_bands(host_ft[g], bands);
+            nft[k][g] = (int)host_ft[g].size();
+            nbt[k][g] = (int)host_bt[g].size();
+            if (int r = upload(&s->ftasks[k][g], &s->ft_cap[k][g], host_ft[g], s->up)) return r;
+            if (int r = upload(&s->btasks[k][g], &s->bt_cap[k][g], host_bt[g], s->up)) return r;
         }
-        concat_force_task_bands(host_ft, bands);
-        nft[k] = (int)host_ft.size();
-        nbt[k] = (int)host_bt.size();
-        if (int r = upload(&s->ftasks[k], &s->ft_cap[k], host_ft, s->up)) return r;
-        return upload(&s->btasks[k], &s->bt_cap[k], host_bt, s->up);
+        return OK;
+    };
+    // One step of list k: each group's force and slot-sum launches, group 1
+    // on s->side between a fork and a join when both groups hold requests.
+    auto launch_step = [&](int k) -> hipError_t {
+        auto group = [&](int g, hipStream_t st) {
+            unsigned* c = s->counter + g * nbody_hip_server::kCounterStride;
+            hipError_t e = launch_force_batch(s->ftasks[k][g], nft[k][g], c, s->blocks, st);
+            if (e == hipSuccess)
+                e = launch_reduce_integrate_batch(s->btasks[k][g], nbt[k][g], s->dt, s->damping, c, st);
+            return e;
+        };
+        if (nft[k][0] == 0 || nft[k][1] == 0) {
+            hipError_t e = group(0, s->stream);
+            return e == hipSuccess ? group(1, s->stream) : e;
+        }
+        hipError_t e = hipEventRecord(s->fork, s->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s->side, s->fork, 0);
+        if (e == hipSuccess) e = group(0, s->stream);
+        if (e == hipSuccess) e = group(1, s->side);
+        if (e == hipSuccess) e = hipEventRecord(s->join, s->side);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s->stream, s->join, 0);
+        return e;
     };
     // The uploads are done: the host copies of the initial conditions can go.
     auto uploaded = [&]() -> int {
@@ -743,9 +787,7 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
         // requests, and builds and uploads the next step's task list into
         // the other list. Only retiring the finished requests waits for the
         // step.
-        hipError_t e = launch_force_batch(s->ftasks[cur], nft[cur], s->counter, s->blocks, s->stream);
-        if (e == hipSuccess)
-            e = launch_reduce_integrate_batch(s->btasks[cur], nbt[cur], s->dt, s->damping, s->counter, s->stream);
+        hipError_t e = launch_step(cur);
         if (e != hipSuccess) {
             set_error("server step: %s", hipGetErrorString(e));
             rc = E_HIP;
@@ -804,6 +846,7 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     // failure a request can sit in more than one list, so each buffer is
     // freed once and forgotten.
     (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->side);
     (void)hipStreamSynchronize(s->up);
     auto drop = [](ServeReq* q) {
         if (q->buf) (void)hipFree(q->buf);
@@ -858,6 +901,7 @@ void nbody_hip_server_destroy(nbody_hip_server* s)
     std::lock_guard<std::mutex> lock(g_mutex);
     if (!s) return;
     (void)hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->side);
     s->release();
     delete s;
 }
