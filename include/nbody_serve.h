@@ -5,8 +5,9 @@
  * `steps` steps of force + integrateGravitation (src/nbody.cu:158-236,293-470).
  * The server keeps every admitted request's positions and velocities resident
  * in HBM and runs continuous batching: each iteration advances every active
- * request by one step with one persistent force launch over all of them (the
- * batched work queue of nbody_engine.h) and one reduce + integrate launch.
+ * request by one step with the batched launches of nbody_engine.h (the
+ * requests in two groups of about equal work, each group one persistent force
+ * launch and one reduce + integrate launch on its own stream).
  * Waiting requests are admitted in arrival order while a slot is free and
  * their device memory fits the HBM budget; while a step runs they are
  * uploaded and join the next step. A request's slot frees when its last step
