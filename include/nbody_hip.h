@@ -95,7 +95,13 @@ float ComputeGravitation_HIP_Host(float *force, float *posMass, float softeningS
 /* Force on bodies [base, base+n) of an N-body system, written to force[0 .. 4n)
  * (local index, as src/nbody_multiGPU_shared.cuh:42-84), on `stream` (a
  * hipStream_t, NULL = null stream), asynchronously. opt may be NULL. Returns 0
- * or a nonzero error code (message in nbody_hip_last_error()). */
+ * or a nonzero error code (message in nbody_hip_last_error()).
+ * Plans with j slices or symmetric blocks keep partial sums in one scratch
+ * buffer per device, shared by all calls: a call on another stream than the
+ * previous user of that buffer first waits (hipStreamWaitEvent) for the
+ * previous user's kernels, so calls on different streams serialise on it.
+ * The buffer grows (is reallocated) when a call needs more, so such a plan is
+ * refused on a stream that is capturing a graph. */
 int nbody_hip_force(float *force, const float *posMass, float softeningSquared, size_t N, size_t base,
                     size_t n, const nbody_hip_options *opt, void *stream);
 

@@ -54,6 +54,7 @@ int device_state(DeviceState** out)
         snprintf(d.arch, sizeof d.arch, "%s", prop.gcnArchName);
         HIP_TRY(hipEventCreate(&d.ev0));
         HIP_TRY(hipEventCreate(&d.ev1));
+        HIP_TRY(hipEventCreateWithFlags(&d.partial_done, hipEventDisableTiming));
         d.init = true;
     }
     *out = &d;
@@ -222,11 +223,23 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
     p.nslices = plan.nslices;
     p.force = force;
     const size_t bytes = plan_scratch_bytes(plan, (int)n);
+    const bool shared = bytes && !scratch;
+    if (shared) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            set_error("this plan needs the library's shared scratch, which cannot be captured into a graph "
+                      "(a later call may reallocate it)");
+            return E_UNSUPPORTED;
+        }
+    }
     if (bytes) {
         HIP_TRY(buf.reserve(bytes));
         p.hi = static_cast<float4*>(buf.ptr);
         p.lo = plan.comp ? p.hi + (size_t)plan.nslices * n : nullptr;
     }
+    if (shared && d.partial_pending && d.partial_stream != stream)
+        HIP_TRY(hipStreamWaitEvent(stream, d.partial_done, 0));
     hipEvent_t before = nullptr, after = nullptr;
     if (timed && d.timing && d.timer_used < kMaxTimedLaunches) {
         if (d.timer_used == d.timer_events.size()) {
@@ -240,6 +253,11 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
         d.timer_used++;
     }
     HIP_TRY(launch_force(plan, p, stream, before, after));
+    if (shared) {
+        HIP_TRY(hipEventRecord(d.partial_done, stream));
+        d.partial_stream = stream;
+        d.partial_pending = true;
+    }
     return OK;
 }
 

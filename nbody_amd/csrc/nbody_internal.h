@@ -90,7 +90,13 @@ struct DeviceState {
     int clock_khz = 0;
     char arch[64] = {0};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    Buffer partial;   // FP32 slice partials
+    Buffer partial;   // FP32 slice partials, shared by the calls that bring no scratch of their own
+    // The last queued use of `partial`: its stream and an event recorded
+    // after its kernels. A call on another stream waits for that event
+    // before its kernels touch the buffer.
+    hipEvent_t partial_done = nullptr;
+    hipStream_t partial_stream = nullptr;
+    bool partial_pending = false;
     Buffer partial64; // FP64 slice partials / energy partials
     Buffer stage_pm, stage_force, small;
     PinnedBuffer host_stage;  // posMass in, force out, for the host-pointer entry
@@ -120,8 +126,11 @@ int make_plan(DeviceState& d, size_t N, size_t n, const nbody_hip_options* opt, 
 int check_args(const void* force, const void* pm, size_t N, size_t base, size_t n);
 // Queue one force evaluation on `stream`. Slice/slot scratch comes from
 // `scratch` (the device's shared buffer when null; a caller that captures the
-// launch into a graph passes its own, already large enough). The kernel timer
-// records only when `timed` is set.
+// launch into a graph passes its own, already large enough). Uses of the
+// shared buffer are ordered across streams (each waits for the previous
+// one's kernels), and refused on a stream that is capturing a graph, since a
+// later, larger call may reallocate the buffer under the graph. The kernel
+// timer records only when `timed` is set.
 int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
               size_t n, hipStream_t stream, Buffer* scratch = nullptr, bool timed = true);
 

@@ -1,0 +1,259 @@
+"""GPU parity at the benchmark workloads themselves, on the states they reach.
+
+- N = 98304 uniform ball (eps^2 = 0.01, dt = 0.016, damping 0.95) stepped 20
+  times on the device: every table entry's force on that final state against
+  the FP64 oracle, on a sample that holds the 256 bodies of smallest |F| of
+  that state (the hardest for a relative bound) and 1792 random ones. The
+  reference recomputes its golden on the same positions every step
+  (src/nbody.cu:313-325,406-418); the same check is made at step 10 too.
+- The 512-system ensemble (N log-uniform in [256, 16384], seed 4): each
+  system's forces of steps 1 and 2 against the FP64 oracle on sampled bodies
+  (its centre-most included), and three systems alone against the mix, bit
+  for bit.
+- A served stream shaped like the served workload (N log-uniform in
+  [1024, 131072], uniform-ball and Plummer ICs, several requests resident at
+  once, one request of at least 100 000 bodies): every final state bit-equal
+  to an ensemble built from the same initial conditions, and the one-step
+  requests' velocities and positions against the oracle's integrator fed the
+  FP64 oracle's force, within the force tolerance carried through one step.
+
+Tolerances: per-body |F - F64| / |F64| max <= 1e-5 and RMS <= 1e-6.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle_lib import force_errors
+
+from nbody_amd import _native, ics, serve
+
+pytestmark = pytest.mark.gpu
+
+TOL_MAX, TOL_RMS = 1e-5, 1e-6
+EPS2 = 0.01
+DT, DAMPING = 0.016, 0.95
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def fp64_all(pm):
+    """FP64 forces of every body from the library's GPU diagnostic (pinned to
+    the oracle by test_gpu_force.test_fp64_force_matches_oracle); used only to
+    find the bodies of smallest |F|, whose forces the oracle then computes."""
+    from gpu_util import dev, host, ptr
+    n = pm.shape[0]
+    out = dev(np.zeros((n, 4), np.float64))
+    d_pm = dev(pm)
+    _native.check_ms(_native.lib().nbody_hip_force_fp64(ptr(out), ptr(d_pm), EPS2, n), "fp64")
+    return host(out)[:, :3]
+
+
+def hard_sample(pm, f64_all, seed, smallest=256, extra=1792):
+    mag = np.linalg.norm(f64_all, axis=1)
+    rng = np.random.default_rng(seed)
+    return np.unique(np.concatenate([np.argsort(mag)[:smallest], rng.choice(pm.shape[0], extra, replace=False)]))
+
+
+class _Sys:
+    def __init__(self, pm, vm):
+        L = _native.lib()
+        self.n = pm.shape[0]
+        self.h = L.nbody_hip_system_create(pm.ctypes.data, vm.ctypes.data, self.n, EPS2, DT, DAMPING, None)
+        assert self.h, _native.last_error()
+
+    def run(self, steps):
+        _native.check(_native.lib().nbody_hip_system_run(self.h, steps, 1, None), "run")
+
+    def state(self):
+        pm = np.zeros((self.n, 4), np.float32)
+        vm = np.zeros((self.n, 4), np.float32)
+        _native.check(_native.lib().nbody_hip_system_read(self.h, pm.ctypes.data, vm.ctypes.data, None), "read")
+        return pm, vm
+
+    def close(self):
+        _native.lib().nbody_hip_system_destroy(self.h)
+
+
+@pytest.fixture(scope="module")
+def c2_states(oracle):
+    """The N = 98304 ball after 10 and after 20 device-resident steps, with a
+    hard sample of each state and the oracle's FP64 forces on it."""
+    n = 98304
+    pm0, vm0 = ics.uniform_ball(n, seed=1)
+    s = _Sys(pm0, vm0)
+    states = {}
+    try:
+        for upto in (10, 20):
+            s.run(10)
+            pm, _ = s.state()
+            idx = hard_sample(pm, fp64_all(pm), seed=upto)
+            states[upto] = (pm, idx, oracle.fp64_forces(pm, EPS2, idx))
+    finally:
+        s.close()
+    return states
+
+
+C2_ENTRIES = ["ComputeGravitation_HIP_Symmetric", "ComputeGravitation_HIP_LDS", "ComputeGravitation_HIP_DPP",
+              "ComputeGravitation_HIP_SGPR", "ComputeGravitation_HIP"]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("entry", C2_ENTRIES)
+@pytest.mark.parametrize("step", [10, 20])
+def test_c2_evolved_state(c2_states, entry, step):
+    from gpu_util import dev, run_entry
+    pm, idx, f64 = c2_states[step]
+    f, ms = run_entry(entry, dev(pm), EPS2)
+    mx, rms = force_errors(f[idx], f64)
+    print(f"{entry} step {step}: {ms:.3f} ms, max {mx:.3e} rms {rms:.3e} on {idx.size} bodies")
+    assert np.all(f[:, 3] == -7.0)
+    assert mx <= TOL_MAX and rms <= TOL_RMS, (entry, step, mx, rms)
+
+
+def _ensemble(systems):
+    L = _native.lib()
+    sizes = np.array([s[0].shape[0] for s in systems], np.uint64)
+    pm = np.ascontiguousarray(np.concatenate([s[0] for s in systems]))
+    vm = np.ascontiguousarray(np.concatenate([s[1] for s in systems]))
+    e2 = np.full(len(systems), EPS2, np.float32)
+    h = L.nbody_hip_ensemble_create(len(systems), sizes.ctypes.data, pm.ctypes.data, vm.ctypes.data,
+                                    e2.ctypes.data, DT, DAMPING)
+    assert h, _native.last_error()
+    return h, sizes, pm.shape[0]
+
+
+def _ensemble_read(h, sizes, total):
+    pm = np.zeros((total, 4), np.float32)
+    vm = np.zeros((total, 4), np.float32)
+    f = np.zeros((total, 4), np.float32)
+    _native.check(_native.lib().nbody_hip_ensemble_read(h, pm.ctypes.data, vm.ctypes.data, f.ctypes.data), "read")
+    cut = np.cumsum(sizes.astype(np.int64))[:-1]
+    return np.split(pm, cut), np.split(vm, cut), np.split(f, cut)
+
+
+def _sample(pm, k, per=32):
+    """The centre-most body, the body of largest |x|, and random others."""
+    n = pm.shape[0]
+    r = np.linalg.norm(pm[:, :3], axis=1)
+    rng = np.random.default_rng(1000 + k)
+    return np.unique(np.concatenate([[np.argmin(r), np.argmax(r)], rng.choice(n, min(n, per - 2), replace=False)]))
+
+
+@pytest.mark.slow
+def test_c4_ensemble_512_systems(oracle):
+    """The ensemble workload as benchmarked: 512 uniform balls, N log-uniform
+    in [256, 16384] (seed 4), batched steps replayed from a graph."""
+    L = _native.lib()
+    sizes = ics.log_uniform_sizes(512, 256, 16384, seed=4)
+    systems = [ics.uniform_ball(int(n), seed=100000 + k) for k, n in enumerate(sizes)]
+    h, sz, total = _ensemble(systems)
+    worst = [0.0, 0.0]
+    try:
+        _native.check(L.nbody_hip_ensemble_run(h, 1, _native.BATCHED_GRAPH, None), "step 1")
+        pm1, vm1, f1 = _ensemble_read(h, sz, total)
+        _native.check(L.nbody_hip_ensemble_run(h, 1, _native.BATCHED_GRAPH, None), "step 2")
+        pm2, vm2, f2 = _ensemble_read(h, sz, total)
+    finally:
+        L.nbody_hip_ensemble_destroy(h)
+    for k, (pm0, _) in enumerate(systems):
+        # step 1's force is the force of the initial state, step 2's of the state after step 1
+        for state, f in ((pm0, f1[k]), (pm1[k], f2[k])):
+            idx = _sample(state, k)
+            mx, rms = force_errors(f[idx], oracle.fp64_forces(state, EPS2, idx))
+            worst = [max(worst[0], mx), max(worst[1], rms)]
+            assert mx <= TOL_MAX and rms <= TOL_RMS, (k, state.shape[0], mx, rms)
+    print(f"512 systems, {total} bodies: worst sampled max {worst[0]:.3e}, rms {worst[1]:.3e}")
+    order = np.argsort(sizes)
+    for k in (int(order[0]), int(order[-1]), 137):
+        ha, sa, ta = _ensemble([systems[k]])
+        try:
+            _native.check(L.nbody_hip_ensemble_run(ha, 2, _native.BATCHED_GRAPH, None), "alone")
+            apm, avm, af = _ensemble_read(ha, sa, ta)
+        finally:
+            L.nbody_hip_ensemble_destroy(ha)
+        for a, m in ((apm[0], pm2[k]), (avm[0], vm2[k]), (af[0], f2[k])):
+            assert np.array_equal(bits(a), bits(m)), (k, int(sizes[k]))
+
+
+def _request_ics(r):
+    if r.ic == _native.IC_PLUMMER:
+        return ics.plummer(int(r.N), seed=int(r.seed))
+    return ics.uniform_ball(int(r.N), seed=int(r.seed))
+
+
+def _served_trace():
+    """Requests shaped like the served workload: N log-uniform in
+    [1024, 131072] (ragged last blocks), both IC kinds, Poisson arrivals at
+    200/s so several are resident at once; one of 100 000+ bodies; steps
+    drawn from [10, 200] but capped at 12 to bound the run, and four
+    one-step requests for the oracle check."""
+    base = serve.poisson_trace(10, 200.0, 1024, 131072, 10, 200, seed=21)
+    reqs = (_native.Request * 12)()
+    for k in range(10):
+        r = base[k]
+        reqs[k] = _native.Request(k, int(r.N), min(int(r.steps), 12), int(r.seed), int(r.ic), EPS2,
+                                  float(r.arrival_s))
+    reqs[3].N = 131072 - 77
+    reqs[3].ic = _native.IC_PLUMMER
+    reqs[6].ic = _native.IC_UNIFORM_BALL
+    reqs[8].ic = _native.IC_PLUMMER
+    for k in (1, 5, 8):
+        reqs[k].steps = 1
+    reqs[10] = _native.Request(10, 1537, 1, 77, _native.IC_UNIFORM_BALL, EPS2, reqs[4].arrival_s)
+    reqs[11] = _native.Request(11, 100003, 4, 78, _native.IC_UNIFORM_BALL, EPS2, reqs[4].arrival_s)
+    return reqs
+
+
+@pytest.mark.slow
+def test_c5_served_trace(oracle):
+    L = _native.lib()
+    reqs = _served_trace()
+    assert any(r.N >= 100000 for r in reqs) and {r.ic for r in reqs} == {0, 1}
+    srv = serve.Server(max_active=64, keep_states=True)
+    try:
+        res, _, _ = srv.run(reqs)
+        assert all(r.status == 0 for r in res), [r.status for r in res]
+        assert srv.peak_active() >= 2
+        final = {k: srv.final_state(k, int(reqs[k].N)) for k in range(len(reqs))}
+    finally:
+        srv.close()
+    # each request against an ensemble of its own ICs stepped the same number
+    # of times (the ensemble is checked against the FP64 oracle above); the
+    # requests with one step count form one ensemble of two or more systems
+    by_steps = {}
+    for k in range(len(reqs)):
+        by_steps.setdefault(int(reqs[k].steps), []).append(k)
+    for steps, ks in by_steps.items():
+        systems = [_request_ics(reqs[k]) for k in ks]
+        h, sz, total = _ensemble(systems)
+        try:
+            _native.check(L.nbody_hip_ensemble_run(h, steps, _native.BATCHED, None), "ensemble")
+            epm, evm, _ = _ensemble_read(h, sz, total)
+        finally:
+            L.nbody_hip_ensemble_destroy(h)
+        for j, k in enumerate(ks):
+            assert np.array_equal(bits(final[k][0]), bits(epm[j])), (k, steps)
+            assert np.array_equal(bits(final[k][1]), bits(evm[j])), (k, steps)
+    # one-step requests: v1 = damping (v0 + invMass F dt), x1 = x0 + v1 dt.
+    # Against the oracle's integrator fed the FP64 force, the velocity may
+    # differ by damping * invMass * dt * |F - F64| plus the roundings of the
+    # fma and the damping product on both sides (at most 4 ulp of v1), and
+    # the position by dt times that plus two roundings of x1.
+    for k in (1, 5, 8, 10):
+        pm0, vm0 = _request_ics(reqs[k])
+        idx = _sample(pm0, k, per=256)
+        f64 = oracle.fp64_forces(pm0, EPS2, idx)
+        f = np.zeros((idx.size, 4), np.float32)
+        f[:, :3] = f64
+        p_ref, v_ref = oracle.integrate_aos(pm0[idx], vm0[idx], f, DT, DAMPING)
+        pm1, vm1 = final[k][0][idx], final[k][1][idx]
+        fmag = np.linalg.norm(f64, axis=1)[:, None]
+        ulp_v = 4 * np.spacing(np.abs(vm1[:, :3]))
+        dv_bound = DAMPING * vm0[idx, 3:4] * DT * TOL_MAX * fmag + ulp_v
+        assert np.all(np.abs(vm1[:, :3] - v_ref[:, :3]) <= dv_bound), k
+        dx_bound = DT * dv_bound + 2 * np.spacing(np.abs(pm1[:, :3]))
+        assert np.all(np.abs(pm1[:, :3] - p_ref[:, :3]) <= dx_bound), k
+        assert np.array_equal(pm1[:, 3], pm0[idx, 3]) and np.array_equal(vm1[:, 3], vm0[idx, 3]), k
