@@ -155,10 +155,13 @@ def _cpu_rate(impl, integ, pm, vm, eps2, dt, damping, budget_s, max_steps=20):
     return float(np.median(times)), len(times), algo, pm
 
 
-def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s, ensemble_sizes=None, served_reqs=None):
+def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s, topology, ensemble_sizes=None, served_reqs=None):
     """The reference's CPU path on this host (rank 0, one GPU only): the
     reference library compiled from its sources when present (kind
-    "reference"), else the oracle's restatement (kind "port")."""
+    "reference"), else the oracle's restatement (kind "port"). `topology` is
+    physical_cores() taken at start-up: once an OpenMP region has run with
+    OMP_PROC_BIND set, the main thread is bound to one CPU and the process's
+    affinity shows only that one."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     import numpy as np
     from nbody_amd import ics
@@ -166,7 +169,7 @@ def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s, ensemble_sizes=N
     ref = ref_cpu()
     kind, impl = ("reference", ref) if ref is not None else ("port", oracle())
     integ = oracle()
-    phys, logical = physical_cores()
+    phys, logical = topology
     omp = _omp()
     env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
     threads = min(phys, env_threads) if env_threads > 0 else phys
@@ -726,6 +729,8 @@ def single_system(args, L, _native, ics, np, torch):
 
 def main():
     args = parse()
+    # the CPU topology before anything binds this thread (see cpu_baseline)
+    topology = physical_cores()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path. "
               "The single-system work would shard by i-range (nbody_hip_force base/n) with an all-gather "
@@ -774,7 +779,7 @@ def main():
         out["plummer"] = run_plummer(args.plummer_steps)
         ens_sizes, _ = ensemble_systems(args.systems)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(pm_h, vm_h, np.float32(0.01), 0.016, 0.95, args.cpu_budget_s,
+        out["cpu_baseline"] = cpu_baseline(pm_h, vm_h, np.float32(0.01), 0.016, 0.95, args.cpu_budget_s, topology,
                                            ensemble_sizes=ens_sizes, served_reqs=reqs)
         if ens_sizes is not None:
             out["ensemble"]["vs_cpu"] = round(out["ensemble"]["value"] / out["cpu_baseline"]["ensemble"]["value"], 1)

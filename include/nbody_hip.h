@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define NBODY_HIP_ABI_VERSION 2
+#define NBODY_HIP_ABI_VERSION 3
 
 /* Variant ids for nbody_hip_options.variant. */
 enum {
@@ -62,9 +62,16 @@ typedef struct nbody_hip_options {
     int variant;        /* NBODY_HIP_* or NBODY_HIP_AUTO */
     int i_per_lane;     /* 1, 2, 4 (symmetric 2, 4, 8), or 0 = auto */
     int jslice;         /* j bodies per slice (symmetric: block size), or 0 = auto */
-    int compensated;    /* 1 TwoSum-compensated sums, 0 plain, -1 auto (on for N > 65536) */
+    int compensated;    /* 1 TwoSum-compensated sums, 0 plain, -1 auto (on for N > 65536);
+                           2 every term added in FP64 and the partial sums kept in FP64, rounded
+                           once to the FP32 force (LDS and symmetric variants; symmetric with
+                           i_per_lane 4 or 8 and blocks of 64 * i_per_lane, default 4) */
     int newton;         /* 1 adds a Newton step after v_rsq_f32 */
     int waves_per_simd; /* symmetric only: 1..8, or 0 = as many as registers and LDS allow */
+    size_t slot_bytes;  /* symmetric only: bytes of block-slot partials one launch may keep, or 0 = 4 GiB.
+                           A system whose slots (12 N^2 / B bytes) need more is evaluated in column
+                           bands of blocks, one launch each, with 24 more bytes per body for the
+                           running sums between them */
 } nbody_hip_options;
 
 /* ---- algorithm-table entries (AOS_GPU contract above) -------------------- */

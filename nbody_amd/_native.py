@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libnbody_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # ensemble modes (include/nbody_engine.h)
 BATCHED_GRAPH, BATCHED, PER_SYSTEM = 0, 1, 2
@@ -29,11 +29,12 @@ class Options(ctypes.Structure):
         ("compensated", ctypes.c_int),
         ("newton", ctypes.c_int),
         ("waves_per_simd", ctypes.c_int),
+        ("slot_bytes", ctypes.c_size_t),
     ]
 
     @classmethod
-    def make(cls, variant=AUTO, i_per_lane=0, jslice=0, compensated=-1, newton=0, waves_per_simd=0):
-        return cls(variant, i_per_lane, jslice, compensated, newton, waves_per_simd)
+    def make(cls, variant=AUTO, i_per_lane=0, jslice=0, compensated=-1, newton=0, waves_per_simd=0, slot_bytes=0):
+        return cls(variant, i_per_lane, jslice, compensated, newton, waves_per_simd, slot_bytes)
 
 
 class Request(ctypes.Structure):

@@ -64,10 +64,9 @@ int device_state(DeviceState** out)
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 // Default symmetric plans aim for at most this many blocks (slot planes of
-// 12 N * blocks bytes), and NBODY_HIP_AUTO leaves the symmetric kernel for the
-// LDS one when the slots would exceed kSymScratchCap bytes.
+// 12 N * blocks bytes; past kSymBandCap they are kept one column band at a
+// time).
 constexpr int kSymSlotsTarget = 512;
-constexpr size_t kSymScratchCap = size_t(8) << 30;
 
 // Default decomposition: enough blocks for ~48 waves per SIMD (the dispatcher
 // balances them across the 256 CUs), j slices of at least 1024 bodies in
@@ -79,6 +78,15 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
     p.newton = opt && opt->newton > 0;
     const int comp_opt = opt ? opt->compensated : -1;
     p.comp = comp_opt < 0 ? (N > 65536) : comp_opt > 0;
+    p.acc64 = comp_opt == 2;
+    if (variant == VAR_SYMMETRIC && p.acc64) {
+        // one wave per block pair, 4 (or 8) bodies per lane, blocks of 64 per lane
+        p.ipl = (opt && opt->i_per_lane > 0) ? opt->i_per_lane : 4;
+        p.jslice = (opt && opt->jslice > 0) ? opt->jslice : 64 * p.ipl;
+        p.nslices = (int)((N + p.jslice - 1) / p.jslice);
+        p.band_cap = opt ? opt->slot_bytes : 0;
+        return p;
+    }
     if (variant == VAR_SYMMETRIC) {
         // blocks of B = 64 * ipl * waves (default 8 bodies per lane and the
         // fewest waves that keep at most kSymSlotsTarget blocks: B = 512 up to
@@ -95,6 +103,7 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
         p.nslices = (int)((N + p.jslice - 1) / p.jslice);
         p.comp = true;
         p.occ = opt ? opt->waves_per_simd : 0;
+        p.band_cap = opt ? opt->slot_bytes : 0;
         return p;
     }
     if (variant == VAR_AOS) {
@@ -124,16 +133,15 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
 // systems of at least two of its blocks (half the pairs, 14-16 VALU + 1 rsq
 // per pair); otherwise the LDS broadcast, whose VALU operands all sit in VGPRs
 // (gfx950 issues those faster than ops reading an SGPR).
+// Largest block count of a symmetric plan (the pair index must fit an int).
+constexpr int kMaxSymBlocks = 16384;
+
 int auto_variant(size_t N, size_t n)
 {
     if (n != N || N < 1024) return VAR_LDS;
-    // the symmetric slots take 12 N^2 / B bytes; above kSymScratchCap use LDS
-    ForcePlan p = default_plan(VAR_SYMMETRIC, N, N, 1, nullptr);
-    return plan_scratch_bytes(p, (int)N) <= kSymScratchCap ? VAR_SYMMETRIC : VAR_LDS;
+    const ForcePlan p = default_plan(VAR_SYMMETRIC, N, N, 1, nullptr);
+    return p.nslices <= kMaxSymBlocks ? VAR_SYMMETRIC : VAR_LDS;
 }
-
-// Largest block count of a symmetric plan (the pair index must fit an int).
-constexpr int kMaxSymBlocks = 16384;
 
 std::tuple<int, size_t, size_t, int, int> tune_key(const nbody_hip_options* opt, size_t N, size_t n)
 {
@@ -145,6 +153,17 @@ std::tuple<int, size_t, size_t, int, int> tune_key(const nbody_hip_options* opt,
 
 int validate_plan(const ForcePlan& p, size_t N)
 {
+    if (p.acc64) {
+        if (p.variant != VAR_LDS && p.variant != VAR_SYMMETRIC) {
+            set_error("FP64 accumulation (compensated = 2) is built for the LDS and symmetric variants");
+            return E_ARG;
+        }
+        if (p.variant == VAR_SYMMETRIC && ((p.ipl != 4 && p.ipl != 8) || p.jslice != 64 * p.ipl)) {
+            set_error("the FP64-accumulation symmetric plan needs i_per_lane 4 or 8 with blocks of 64*i_per_lane "
+                      "(got %d, %d)", p.ipl, p.jslice);
+            return E_ARG;
+        }
+    }
     if (p.variant == VAR_SYMMETRIC) {
         if (!symmetric_shape_ok(p.ipl, p.jslice)) {
             set_error("symmetric plan needs i_per_lane 2 with blocks of 128 or 512 bodies, or 4 or 8 with "
@@ -181,6 +200,7 @@ int make_plan(DeviceState& d, size_t N, size_t n, const nbody_hip_options* opt, 
     if (!explicit_shape) {
         auto it = d.tuned.find(tune_key(opt, N, n));
         if (it != d.tuned.end()) p = it->second;
+        if (p.variant == VAR_SYMMETRIC && opt) p.band_cap = opt->slot_bytes;
     }
     if (int r = validate_plan(p, N)) return r;
     *plan = p;
@@ -206,9 +226,9 @@ int check_args(const void* force, const void* pm, size_t N, size_t base, size_t 
 
 // Queue one force evaluation on `stream` (caller holds g_mutex).
 int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
-              size_t n, hipStream_t stream, Buffer* scratch, bool timed)
+              size_t n, hipStream_t stream, Buffer* own, bool timed)
 {
-    Buffer& buf = scratch ? *scratch : d.partial;
+    Buffer& buf = own ? *own : d.partial;
     if (plan.variant == VAR_SYMMETRIC && (base != 0 || n != N)) {
         set_error("the symmetric variant computes whole systems only (base = 0, n = N)");
         return E_UNSUPPORTED;
@@ -223,7 +243,7 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
     p.nslices = plan.nslices;
     p.force = force;
     const size_t bytes = plan_scratch_bytes(plan, (int)n);
-    const bool shared = bytes && !scratch;
+    const bool shared = bytes && !own;
     if (shared) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         HIP_TRY(hipStreamIsCapturing(stream, &cs));
@@ -235,8 +255,12 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
     }
     if (bytes) {
         HIP_TRY(buf.reserve(bytes));
-        p.hi = static_cast<float4*>(buf.ptr);
-        p.lo = plan.comp ? p.hi + (size_t)plan.nslices * n : nullptr;
+        if (plan.acc64) {
+            p.hi64 = static_cast<double*>(buf.ptr);
+        } else {
+            p.hi = static_cast<float4*>(buf.ptr);
+            p.lo = plan.comp ? p.hi + (size_t)plan.nslices * n : nullptr;
+        }
     }
     if (shared && d.partial_pending && d.partial_stream != stream)
         HIP_TRY(hipStreamWaitEvent(stream, d.partial_done, 0));
@@ -507,9 +531,10 @@ int nbody_hip_plan(size_t N, size_t n, const nbody_hip_options* opt, nbody_hip_o
     out->variant = p.variant;
     out->i_per_lane = p.ipl;
     out->jslice = p.jslice;
-    out->compensated = p.comp ? 1 : 0;
+    out->compensated = p.acc64 ? 2 : p.comp ? 1 : 0;
     out->newton = p.newton ? 1 : 0;
     out->waves_per_simd = p.occ;
+    out->slot_bytes = p.band_cap;
     return OK;
 }
 
@@ -541,10 +566,17 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
                 ForcePlan c = default_plan(fam, N, N, d->cus, &o);
                 if (c.nslices > kMaxSymBlocks) continue;
                 cands.push_back(c);
-                // the same shape with one and two fewer resident waves per SIMD
+                // the same shape with one and two fewer resident waves per
+                // SIMD, and with two: a launch of a few tasks per SIMD ends
+                // unevenly when they all start at once, while at two resident
+                // waves (about the issue rate of six) the later tasks start
+                // as slots free
                 const int natural = symmetric_natural_occupancy(c);
-                for (int less = 1; less <= 2 && natural - less >= 3; ++less) {
-                    c.occ = natural - less;
+                int last = natural;
+                for (int waves : {natural - 1, natural - 2, 2}) {
+                    if (waves < 2 || waves >= last) continue;
+                    last = waves;
+                    c.occ = waves;
                     cands.push_back(c);
                 }
             }

@@ -36,7 +36,8 @@ __global__ __launch_bounds__(kWave) void k_force_batch(const ForceTask* __restri
         if (t >= ntasks) return;
         const ForceTask task = tasks[t];
         if (task.kind == TASK_PAIR) {
-            sym_pair<kPairA, 1, false, true>(task.pm, task.n, task.npad, task.eps2, task.part, task.a, task.b, jacc);
+            sym_pair<kPairA, 1, false, true>(task.pm, task.n, task.eps2, SlotMap::whole(task.part, task.nb, task.npad),
+                                             task.a, task.b, jacc);
             continue;
         }
         if (task.kind == TASK_DIAG) {

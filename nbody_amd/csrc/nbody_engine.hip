@@ -299,9 +299,10 @@ int nbody_hip_system_plan(const nbody_hip_system* s, nbody_hip_options* plan)
     plan->variant = s->plan.variant;
     plan->i_per_lane = s->plan.ipl;
     plan->jslice = s->plan.jslice;
-    plan->compensated = s->plan.comp ? 1 : 0;
+    plan->compensated = s->plan.acc64 ? 2 : s->plan.comp ? 1 : 0;
     plan->newton = s->plan.newton ? 1 : 0;
     plan->waves_per_simd = s->plan.occ;
+    plan->slot_bytes = s->plan.band_cap;
     return OK;
 }
 

@@ -125,14 +125,14 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
 int make_plan(DeviceState& d, size_t N, size_t n, const nbody_hip_options* opt, ForcePlan* plan);
 int check_args(const void* force, const void* pm, size_t N, size_t base, size_t n);
 // Queue one force evaluation on `stream`. Slice/slot scratch comes from
-// `scratch` (the device's shared buffer when null; a caller that captures the
+// `own` (the device's shared buffer when null; a caller that captures the
 // launch into a graph passes its own, already large enough). Uses of the
 // shared buffer are ordered across streams (each waits for the previous
 // one's kernels), and refused on a stream that is capturing a graph, since a
 // later, larger call may reallocate the buffer under the graph. The kernel
 // timer records only when `timed` is set.
 int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
-              size_t n, hipStream_t stream, Buffer* scratch = nullptr, bool timed = true);
+              size_t n, hipStream_t stream, Buffer* own = nullptr, bool timed = true);
 
 }  // namespace detail
 }  // namespace nbody

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <utility>
 #include <vector>
 
 namespace nbody {
@@ -27,9 +28,20 @@ struct ForcePlan {
     int jslice = 0;     // j bodies per slice (symmetric: bodies per block)
     int nslices = 1;    // ceil(n / jslice)
     bool comp = false;  // TwoSum-compensated accumulation
+    bool acc64 = false; // every term added in FP64, partial sums kept in FP64 (LDS and symmetric)
     bool newton = false;
     int occ = 0;        // symmetric: waves per SIMD cap (0 = register-limited occupancy)
+    size_t band_cap = 0;  // symmetric: bytes of slot partials one band may keep (0 = kSymBandCap)
 };
+
+// Bytes of block-slot partials one launch of the symmetric plan keeps by
+// default; a system whose slots need more runs in several column bands.
+constexpr size_t kSymBandCap = size_t(4) << 30;
+
+// The column bands [q0, q1) of a symmetric plan with nb blocks of B bodies
+// whose partials (elem bytes each) fit in cap bytes each (a band of one
+// column always runs).
+std::vector<std::pair<int, int>> symmetric_bands(int nb, int B, size_t cap, int elem = 4);
 
 struct ForceParams {
     const float4* pm = nullptr;  // posMass (x, y, z, m) of the whole system: j source and i source
@@ -41,6 +53,7 @@ struct ForceParams {
     int nslices = 1;
     float4* hi = nullptr;        // slice partials [nslices][ni]; nullptr -> write force directly
     float4* lo = nullptr;        // compensation partials [nslices][ni] (comp only)
+    double* hi64 = nullptr;      // FP64 partials (acc64): slices [nslices][3][ni], or the symmetric slot planes
     float* force = nullptr;      // 4 floats per i body; x, y, z written, w left alone
 };
 

@@ -12,11 +12,13 @@
 //     on the plan only, never on scheduling.
 // With one slice the kernel writes force directly. Force records are written
 // x, y, z only; w is left alone (the contract of src/nbody.cu:351-371).
+#include "nbody_acc64.h"
 #include "nbody_device.h"
 #include "nbody_kernels.h"
 #include "nbody_sym.h"
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -140,6 +142,64 @@ __global__ __launch_bounds__(kBlock) void k_force_lds(ForceParams p)
     store<I>(p, hi, COMP ? &lo : nullptr);
 }
 
+// k_force_lds with every term added in FP64 (the accurate plan): the same
+// tiles and term inputs, FP64 accumulators, FP64 slice partials
+// hi64[slice][component][ni] (or the force, rounded once, with one slice).
+template <int I, bool NEWTON>
+__global__ __launch_bounds__(kBlock) void k_force_lds64(ForceParams p)
+{
+    __shared__ float4 tile[2][kBlock];
+    Lanes<I> me;
+    Lanes64<I> acc;
+    load_i<I>(p, me);
+    zero<I>(acc);
+    const int j0 = blockIdx.y * p.jslice;
+    const int len = min(p.n - j0, p.jslice);
+    const float4* __restrict__ src = p.pm + j0;
+    int buf = 0;
+    for (int t = 0; t < len; t += kBlock, buf ^= 1) {
+        const int k = t + threadIdx.x;
+        tile[buf][threadIdx.x] = k < len ? src[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        const int cnt = min(kBlock, len - t);
+        const float4* t4 = tile[buf];
+        for (int q = 0; q < cnt; ++q) {
+            const float4 b = t4[q];
+#pragma unroll
+            for (int u = 0; u < I; ++u) {
+                const float dx = b.x - me.x[u];
+                const float dy = b.y - me.y[u];
+                const float dz = b.z - me.z[u];
+                float d2 = __builtin_fmaf(dx, dx, p.eps2);
+                d2 = __builtin_fmaf(dy, dy, d2);
+                d2 = __builtin_fmaf(dz, dz, d2);
+                const float inv = rsqrt_f32<NEWTON>(d2);
+                const double w = (double)((b.w * inv) * (inv * inv));
+                acc.x[u] = __builtin_fma((double)dx, w, acc.x[u]);
+                acc.y[u] = __builtin_fma((double)dy, w, acc.y[u]);
+                acc.z[u] = __builtin_fma((double)dz, w, acc.z[u]);
+            }
+        }
+    }
+    const int mbase = blockIdx.x * (kBlock * I) + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < I; ++u) {
+        const int m = mbase + u * kBlock;
+        if (m >= p.ni) continue;
+        if (p.hi64 == nullptr) {
+            float* f = p.force + 4 * (size_t)m;
+            f[0] = (float)acc.x[u];
+            f[1] = (float)acc.y[u];
+            f[2] = (float)acc.z[u];
+        } else {
+            double* o = p.hi64 + (size_t)blockIdx.y * 3 * p.ni + m;
+            o[0] = acc.x[u];
+            o[p.ni] = acc.y[u];
+            o[2 * (size_t)p.ni] = acc.z[u];
+        }
+    }
+}
+
 template <int I, bool NEWTON, int K>
 __device__ __forceinline__ void dpp_steps(const float4& b, float eps2, const Lanes<I>& me, Lanes<I>& acc)
 {
@@ -208,13 +268,15 @@ __global__ __launch_bounds__(kBlock) void k_force_dpp(ForceParams p)
 // end of the launch evenly (one task per diagonal block, each pair once,
 // halves their work but measured 2.5 % slower at N = 98304 for its coarser
 // end; the batched kernel, whose queue mixes many systems, uses it).
+// One launch covers the column band q0 <= Q < q1 of the block triangle: the
+// pairs (P, Q), P < Q, then A pieces of each band block's diagonal.
 struct SymParams {
     const float4* pm;
     int n;
-    int nb;      // blocks
-    int npad;    // nb * B
+    int q0;        // band: first column block
+    int npairs;    // pairs in the band: Q(Q-1)/2 summed over its columns
     float eps2;
-    float* part; // three planes [3][nb][npad]
+    SlotMap slots; // where the band's partials go
 };
 
 template <int A, int W, bool NEWTON>
@@ -229,34 +291,57 @@ __global__ __launch_bounds__(64 * W) void k_force_sym(SymParams p)
     // SIMD (121 TFLOP/s credited at N = 98304 against 115), so it claims v79:
     // 80 VGPRs, 6 waves on every SIMD.
     if constexpr (A == 8 && W == 1) asm volatile("" ::: "v79");
-    const int npairs = p.nb * (p.nb - 1) / 2;
     const int t = blockIdx.x;
-    if (t < npairs) {
+    if (t < p.npairs) {
         int P, Q;
-        upper_pair(t, P, Q);
-        sym_pair<A, W, NEWTON>(p.pm, p.n, p.npad, p.eps2, p.part, P, Q, jacc);
+        upper_pair(p.q0 * (p.q0 - 1) / 2 + t, P, Q);
+        sym_pair<A, W, NEWTON>(p.pm, p.n, p.eps2, p.slots, P, Q, jacc);
     } else {
-        const int d = t - npairs;
-        sym_diag_piece<B, NEWTON, false>(p.pm, p.n, p.npad, p.eps2, p.part, d / A, d % A, 64 * W);
+        const int d = t - p.npairs;
+        sym_diag_piece<B, NEWTON, false>(p.pm, p.n, p.eps2, p.slots, p.q0 + d / A, d % A, 64 * W);
     }
 }
 
-// Block slots summed in slot order with TwoSum, rounded once. One lane per
-// (body, component) (blockIdx.y = component); slots are fetched eight at a
-// time so the loads overlap, then folded strictly in slot order.
-__global__ __launch_bounds__(kBlock) void k_reduce_slots(const float* __restrict__ part, int nb, int npad, int n,
-                                                         float* __restrict__ force)
+// The band's slots of each body summed in slot order with TwoSum. One lane
+// per (body, component) (blockIdx.y = component); slots are fetched eight at
+// a time so the loads overlap, then folded strictly in slot order. With one
+// band the sum is rounded once into the force. With several, the running
+// (sum, error) pair of each body carries from band to band in `state`
+// ([2][3][n]; the first band starts it, a body of a block past the band adds
+// nothing yet) and the last band rounds it into the force; each body's slots
+// are then folded in band order, the same fixed order on every run.
+__global__ __launch_bounds__(kBlock) void k_reduce_band(SlotMap m, int n, int B, int q0, int q1, int nb,
+                                                        float* __restrict__ state, float* __restrict__ force)
 {
     constexpr int G = 8;
     const int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= n) return;
-    const float* src = part + (size_t)blockIdx.y * nb * npad + b;
+    const int c = blockIdx.y;
+    const bool first = q0 == 0, last = q1 == nb;
+    const int blk = b / B;
+    if (!first && blk >= q1) return;
+    const float* src = nullptr;
+    int slots = 0;
+    size_t row = 0;
+    if (blk < q0) {
+        src = m.a + c * m.aplane + b;
+        slots = q1 - q0;
+        row = m.arow;
+    } else if (blk < q1) {
+        src = m.b + c * m.bplane + (b - (size_t)q0 * B);
+        slots = q1;
+        row = m.brow;
+    }
     float s = 0.f, e = 0.f;
+    if (!first) {
+        s = state[(size_t)c * n + b];
+        e = state[(size_t)(3 + c) * n + b];
+    }
     int k = 0;
-    for (; k + G <= nb; k += G) {
+    for (; k + G <= slots; k += G) {
         float v[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) v[g] = src[(size_t)(k + g) * npad];
+        for (int g = 0; g < G; ++g) v[g] = src[(size_t)(k + g) * row];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             float err;
@@ -264,12 +349,100 @@ __global__ __launch_bounds__(kBlock) void k_reduce_slots(const float* __restrict
             e += err;
         }
     }
-    for (; k < nb; ++k) {
+    for (; k < slots; ++k) {
         float err;
-        s = two_sum(s, src[(size_t)k * npad], err);
+        s = two_sum(s, src[(size_t)k * row], err);
         e += err;
     }
-    force[4 * (size_t)b + blockIdx.y] = s + e;
+    if (last) {
+        force[4 * (size_t)b + c] = s + e;
+    } else {
+        state[(size_t)c * n + b] = s;
+        state[(size_t)(3 + c) * n + b] = e;
+    }
+}
+
+// The symmetric plan with every term added in FP64 (the accurate plan): one
+// wave per task, A bodies per lane, blocks of B = 64 A; slots are doubles.
+struct SymParams64 {
+    const float4* pm;
+    int n;
+    int q0;
+    int npairs;
+    float eps2;
+    SlotMap64 slots;
+};
+
+template <int A, bool NEWTON>
+__global__ __launch_bounds__(64) void k_force_sym64(SymParams64 p)
+{
+    const int t = blockIdx.x;
+    if (t < p.npairs) {
+        int P, Q;
+        upper_pair(p.q0 * (p.q0 - 1) / 2 + t, P, Q);
+        sym_pair64<A, NEWTON>(p.pm, p.n, p.eps2, p.slots, P, Q);
+    } else {
+        const int d = t - p.npairs;
+        sym_diag_piece64<64 * A, NEWTON>(p.pm, p.n, p.eps2, p.slots, p.q0 + d / A, d % A);
+    }
+}
+
+// k_reduce_band for FP64 slots: each body's slots of the band added in slot
+// order in FP64, the running sum carried between bands in `state` ([3][n]
+// doubles), rounded once into the force by the last band.
+__global__ __launch_bounds__(kBlock) void k_reduce_band64(SlotMap64 m, int n, int B, int q0, int q1, int nb,
+                                                          double* __restrict__ state, float* __restrict__ force)
+{
+    constexpr int G = 8;
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= n) return;
+    const int c = blockIdx.y;
+    const bool first = q0 == 0, last = q1 == nb;
+    const int blk = b / B;
+    if (!first && blk >= q1) return;
+    const double* src = nullptr;
+    int slots = 0;
+    size_t row = 0;
+    if (blk < q0) {
+        src = m.a + c * m.aplane + b;
+        slots = q1 - q0;
+        row = m.arow;
+    } else if (blk < q1) {
+        src = m.b + c * m.bplane + (b - (size_t)q0 * B);
+        slots = q1;
+        row = m.brow;
+    }
+    double s = first ? 0.0 : state[(size_t)c * n + b];
+    int k = 0;
+    for (; k + G <= slots; k += G) {
+        double v[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) v[g] = src[(size_t)(k + g) * row];
+#pragma unroll
+        for (int g = 0; g < G; ++g) s += v[g];
+    }
+    for (; k < slots; ++k) s += src[(size_t)k * row];
+    if (last) force[4 * (size_t)b + c] = (float)s;
+    else state[(size_t)c * n + b] = s;
+}
+
+// FP64 slice partials [slice][component][ni] added in slice order, rounded once.
+__global__ __launch_bounds__(kBlock) void k_reduce64(const double* __restrict__ hi64, int nslices, int ni,
+                                                     float* __restrict__ force)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= ni) return;
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (int s = 0; s < nslices; ++s) {
+        const double* o = hi64 + (size_t)s * 3 * ni + m;
+        x += o[0];
+        y += o[ni];
+        z += o[2 * (size_t)ni];
+    }
+    float* f = force + 4 * (size_t)m;
+    f[0] = (float)x;
+    f[1] = (float)y;
+    f[2] = (float)z;
 }
 
 // Slice partials summed in slice order; compensated plans merge the TwoSum
@@ -494,15 +667,31 @@ const void* sym_kernel()
 }
 
 template <int A, int W>
-hipError_t go_sym(const ForcePlan& plan, const SymParams& sp, hipStream_t s)
+hipError_t go_sym(const ForcePlan& plan, const SymParams& sp, int band_blocks, hipStream_t s)
 {
-    const int tasks = sp.nb * (sp.nb - 1) / 2 + sp.nb * A;
+    const int tasks = sp.npairs + band_blocks * A;
     const void* k = plan.newton ? sym_kernel<A, W, true>() : sym_kernel<A, W, false>();
     const size_t dyn = plan.occ > 0 ? lds_for_occupancy(k, 64 * W, plan.occ) : 0;
     if (plan.newton) k_force_sym<A, W, true><<<tasks, 64 * W, dyn, s>>>(sp);
     else k_force_sym<A, W, false><<<tasks, 64 * W, dyn, s>>>(sp);
     return hipGetLastError();
 }
+
+hipError_t go_sym_shape(const ForcePlan& plan, const SymParams& sp, int band_blocks, hipStream_t s)
+{
+    const int w = plan.jslice / (64 * plan.ipl);
+    if (plan.ipl == 2) return w == 1 ? go_sym<2, 1>(plan, sp, band_blocks, s) : go_sym<2, 4>(plan, sp, band_blocks, s);
+    if (plan.ipl == 4)
+        return w == 1   ? go_sym<4, 1>(plan, sp, band_blocks, s)
+               : w == 2 ? go_sym<4, 2>(plan, sp, band_blocks, s)
+                        : go_sym<4, 4>(plan, sp, band_blocks, s);
+    return w == 1   ? go_sym<8, 1>(plan, sp, band_blocks, s)
+           : w == 2 ? go_sym<8, 2>(plan, sp, band_blocks, s)
+                    : go_sym<8, 4>(plan, sp, band_blocks, s);
+}
+
+// Floats of slot regions band [q0, q1) keeps: region A then region B.
+size_t band_floats(int q0, int q1, int B) { return (size_t)3 * (q1 - q0) * B * ((size_t)q0 + q1); }
 
 template <int A, int W>
 int natural_occupancy(bool newton)
@@ -532,11 +721,35 @@ int symmetric_natural_occupancy(const ForcePlan& plan)
                                                                   : natural_occupancy<8, 4>(plan.newton);
 }
 
+std::vector<std::pair<int, int>> symmetric_bands(int nb, int B, size_t cap, int elem)
+{
+    std::vector<std::pair<int, int>> bands;
+    const double per_col = 3.0 * elem * B;  // bytes per band column and block below q1, both regions
+    for (int q0 = 0; q0 < nb;) {
+        // the widest band [q0, q0 + w) whose regions, 3 elem B w (2 q0 + w) bytes, fit the cap
+        const double w = -(double)q0 + std::sqrt((double)q0 * q0 + (double)cap / per_col);
+        int width = std::max(1, std::min(nb - q0, (int)w));
+        while (width > 1 && band_floats(q0, q0 + width, B) * elem > cap) --width;
+        bands.emplace_back(q0, q0 + width);
+        q0 += width;
+    }
+    return bands;
+}
+
 size_t plan_scratch_bytes(const ForcePlan& plan, int ni)
 {
-    if (plan.variant == VAR_SYMMETRIC)
-        return (size_t)3 * plan.nslices * ((size_t)plan.nslices * plan.jslice) * sizeof(float);
+    if (plan.variant == VAR_SYMMETRIC) {
+        const int elem = plan.acc64 ? 8 : 4;
+        const auto bands =
+            symmetric_bands(plan.nslices, plan.jslice, plan.band_cap ? plan.band_cap : kSymBandCap, elem);
+        size_t most = 0;
+        for (const auto& b : bands) most = std::max(most, band_floats(b.first, b.second, plan.jslice));
+        // the running sums carried between bands: (sum, error) float pairs or one double per component
+        const size_t carry = bands.size() > 1 ? (size_t)6 * ni * sizeof(float) : 0;
+        return most * elem + carry;
+    }
     if (plan.nslices <= 1) return 0;
+    if (plan.acc64) return (size_t)plan.nslices * ni * 3 * sizeof(double);
     return (size_t)plan.nslices * ni * sizeof(float4) * (plan.comp ? 2 : 1);
 }
 
@@ -559,30 +772,102 @@ hipError_t launch_force(const ForcePlan& plan, const ForceParams& p, hipStream_t
         if (e == hipSuccess && after) e = hipEventRecord(after, stream);
         return e;
     }
+    if (plan.variant == VAR_SYMMETRIC && plan.acc64) {
+        if (p.i0 != 0 || p.ni != p.n || plan.jslice != 64 * plan.ipl || (plan.ipl != 4 && plan.ipl != 8) ||
+            p.hi64 == nullptr)
+            return hipErrorInvalidValue;
+        const int B = plan.jslice, nb = plan.nslices;
+        const auto bands = symmetric_bands(nb, B, plan.band_cap ? plan.band_cap : kSymBandCap, 8);
+        double* base = p.hi64;
+        size_t most = 0;
+        for (const auto& b : bands) most = std::max(most, band_floats(b.first, b.second, B));
+        double* state = bands.size() > 1 ? base + most : nullptr;
+        for (size_t k = 0; k < bands.size(); ++k) {
+            const int q0 = bands[k].first, q1 = bands[k].second, w = q1 - q0;
+            SymParams64 sp;
+            sp.pm = p.pm;
+            sp.n = p.n;
+            sp.q0 = q0;
+            sp.npairs = (int)(((long long)q1 * (q1 - 1) - (long long)q0 * (q0 - 1)) / 2);
+            sp.eps2 = p.eps2;
+            sp.slots.a = base;
+            sp.slots.arow = q0 * B;
+            sp.slots.aplane = (size_t)w * q0 * B;
+            sp.slots.b = base + 3 * sp.slots.aplane;
+            sp.slots.brow = w * B;
+            sp.slots.bplane = (size_t)q1 * w * B;
+            sp.slots.q0 = q0;
+            const int tasks = sp.npairs + w * plan.ipl;
+            if (plan.ipl == 4) {
+                if (plan.newton) k_force_sym64<4, true><<<tasks, 64, 0, stream>>>(sp);
+                else k_force_sym64<4, false><<<tasks, 64, 0, stream>>>(sp);
+            } else {
+                if (plan.newton) k_force_sym64<8, true><<<tasks, 64, 0, stream>>>(sp);
+                else k_force_sym64<8, false><<<tasks, 64, 0, stream>>>(sp);
+            }
+            e = hipGetLastError();
+            if (e == hipSuccess && after && k + 1 == bands.size()) e = hipEventRecord(after, stream);
+            if (e != hipSuccess) return e;
+            k_reduce_band64<<<dim3((p.n + kBlock - 1) / kBlock, 3), kBlock, 0, stream>>>(sp.slots, p.n, B, q0, q1,
+                                                                                        nb, state, p.force);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
     if (plan.variant == VAR_SYMMETRIC) {
         if (p.i0 != 0 || p.ni != p.n || !symmetric_shape_ok(plan.ipl, plan.jslice) || p.hi == nullptr)
             return hipErrorInvalidValue;
-        SymParams sp;
-        sp.pm = p.pm;
-        sp.n = p.n;
-        sp.nb = plan.nslices;
-        sp.npad = plan.nslices * plan.jslice;
-        sp.eps2 = p.eps2;
-        sp.part = reinterpret_cast<float*>(p.hi);
-        const int w = plan.jslice / (64 * plan.ipl);
-        if (plan.ipl == 2) e = w == 1 ? go_sym<2, 1>(plan, sp, stream) : go_sym<2, 4>(plan, sp, stream);
-        else if (plan.ipl == 4) e = w == 1 ? go_sym<4, 1>(plan, sp, stream) : w == 2 ? go_sym<4, 2>(plan, sp, stream)
-                                                                               : go_sym<4, 4>(plan, sp, stream);
-        else e = w == 1 ? go_sym<8, 1>(plan, sp, stream) : w == 2 ? go_sym<8, 2>(plan, sp, stream)
-                                                                  : go_sym<8, 4>(plan, sp, stream);
-        if (e == hipSuccess && after) e = hipEventRecord(after, stream);
-        if (e != hipSuccess) return e;
-        k_reduce_slots<<<dim3((p.n + kBlock - 1) / kBlock, 3), kBlock, 0, stream>>>(sp.part, sp.nb, sp.npad, p.n,
-                                                                                   p.force);
-        return hipGetLastError();
+        const int B = plan.jslice, nb = plan.nslices;
+        const auto bands = symmetric_bands(nb, B, plan.band_cap ? plan.band_cap : kSymBandCap);
+        float* base = reinterpret_cast<float*>(p.hi);
+        size_t most = 0;
+        for (const auto& b : bands) most = std::max(most, band_floats(b.first, b.second, B));
+        float* state = bands.size() > 1 ? base + most : nullptr;
+        for (size_t k = 0; k < bands.size(); ++k) {
+            const int q0 = bands[k].first, q1 = bands[k].second, w = q1 - q0;
+            SymParams sp;
+            sp.pm = p.pm;
+            sp.n = p.n;
+            sp.q0 = q0;
+            sp.npairs = (int)(((long long)q1 * (q1 - 1) - (long long)q0 * (q0 - 1)) / 2);
+            sp.eps2 = p.eps2;
+            sp.slots.a = base;
+            sp.slots.arow = q0 * B;
+            sp.slots.aplane = (size_t)w * q0 * B;
+            sp.slots.b = base + 3 * sp.slots.aplane;
+            sp.slots.brow = w * B;
+            sp.slots.bplane = (size_t)q1 * w * B;
+            sp.slots.q0 = q0;
+            e = go_sym_shape(plan, sp, w, stream);
+            if (e == hipSuccess && after && k + 1 == bands.size()) e = hipEventRecord(after, stream);
+            if (e != hipSuccess) return e;
+            k_reduce_band<<<dim3((p.n + kBlock - 1) / kBlock, 3), kBlock, 0, stream>>>(sp.slots, p.n, B, q0, q1, nb,
+                                                                                      state, p.force);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     }
     const int per_block = kBlock * plan.ipl;
     const dim3 g((p.ni + per_block - 1) / per_block, p.nslices);
+    if (plan.acc64) {
+        if (plan.variant != VAR_LDS || (p.nslices > 1 && p.hi64 == nullptr)) return hipErrorInvalidValue;
+        switch (plan.ipl) {
+        case 1: plan.newton ? k_force_lds64<1, true><<<g, kBlock, 0, stream>>>(p)
+                            : k_force_lds64<1, false><<<g, kBlock, 0, stream>>>(p); break;
+        case 2: plan.newton ? k_force_lds64<2, true><<<g, kBlock, 0, stream>>>(p)
+                            : k_force_lds64<2, false><<<g, kBlock, 0, stream>>>(p); break;
+        case 4: plan.newton ? k_force_lds64<4, true><<<g, kBlock, 0, stream>>>(p)
+                            : k_force_lds64<4, false><<<g, kBlock, 0, stream>>>(p); break;
+        default: return hipErrorInvalidValue;
+        }
+        e = hipGetLastError();
+        if (e == hipSuccess && after) e = hipEventRecord(after, stream);
+        if (e != hipSuccess || p.hi64 == nullptr) return e;
+        k_reduce64<<<(p.ni + kBlock - 1) / kBlock, kBlock, 0, stream>>>(p.hi64, p.nslices, p.ni, p.force);
+        return hipGetLastError();
+    }
     switch (plan.variant) {
     case VAR_SGPR: e = go_family<FamSgpr>(plan, g, stream, p); break;
     case VAR_LDS: e = go_family<FamLds>(plan, g, stream, p); break;

@@ -24,11 +24,53 @@
 // side of (P, Q), slot P for its j side, slot P for the diagonal. The partials
 // live in three float planes part[3][nb][npad] (npad = nb*B) and are added in
 // slot order with TwoSum by the reduction kernels.
+//
+// A large system is evaluated in column bands of blocks Q in [q0, q1), one
+// launch each, so that the partials of one band are all that is kept
+// (SlotMap): a body of block P < q0 has one slot per band column (region A,
+// [3][q1-q0][q0*B]); a body of a band block has one slot per block below q1
+// (region B, [3][q1][(q1-q0)*B]): the j sides of the pairs (P, Q), its
+// diagonal, and the i sides of its pairs with the band's later blocks. One
+// band over all columns (q0 = 0) is exactly the layout above.
 #pragma once
 
 #include "nbody_device.h"
 
 namespace nbody {
+
+// Where the partials of one band go (see above), as elements of type T
+// (float; double for the FP64-accumulation kernels). Plane c of a region
+// starts c * plane elements after its base; a slot row holds `row` elements.
+template <class T>
+struct SlotMapT {
+    T* a = nullptr;  // region A: bodies of blocks below q0, slot Q - q0
+    T* b = nullptr;  // region B: bodies of blocks q0 .. q1-1, slot = the other block
+    size_t aplane = 0, bplane = 0;
+    int arow = 0, brow = 0;
+    int q0 = 0;
+
+    // One band over every column of a system: planes [3][nb][npad].
+    __host__ __device__ static SlotMapT whole(T* part, int nb, int npad)
+    {
+        SlotMapT m;
+        m.b = part;
+        m.brow = npad;
+        m.bplane = (size_t)nb * npad;
+        return m;
+    }
+    // The i-side partials of block P's bodies against block Q (slot Q), B bodies per block.
+    __device__ __forceinline__ T* i_side(int P, int Q, int B) const
+    {
+        return P < q0 ? a + (size_t)(Q - q0) * arow + (size_t)P * B : b + (size_t)Q * brow + (size_t)(P - q0) * B;
+    }
+    __device__ __forceinline__ size_t i_plane(int P) const { return P < q0 ? aplane : bplane; }
+    // The j-side partials of block Q's bodies against block P (slot P); Q is a band block.
+    __device__ __forceinline__ T* j_side(int P, int Q, int B) const
+    {
+        return b + (size_t)P * brow + (size_t)(Q - q0) * B;
+    }
+};
+typedef SlotMapT<float> SlotMap;
 
 // Every lane reads the lane below it, lane 0 reads lane 63 (DPP wave_ror:1).
 // All source lanes are valid, so no "old" value has to be kept.
@@ -371,8 +413,13 @@ __device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, 
                                                const float (&mi)[8], float eps2, unsigned long long mask)
 {
     float dx, dy, dz, s, t;
+    // The rotation-only steps (tile 0 outside its own sub-block) begin with a
+    // DPP read; its source may have been written by a VALU op just before
+    // the block (the j load's select, the zeroed reactions), and gfx9 needs
+    // two wait states between the two, which the compiler does not see
+    // inside assembly: s_nop 1 provides them.
     if constexpr (!EQ && T == 0 && MODE == 0)
-        asm volatile(NBODY_LT0(NBODY_SYM_PAIR) NBODY_ROR_GEN NBODY_DIAG_OPS);
+        asm volatile("s_nop 1\n" NBODY_ROR_GEN NBODY_DIAG_OPS);
     else if constexpr (!EQ && T == 0 && MODE == 1)
         asm volatile(NBODY_LT0(NBODY_SYM_PAIR) NBODY_SYM_PAIR(0) NBODY_ROR_GEN NBODY_DIAG_OPS);
     else if constexpr (!EQ && T == 0 && MODE == 2)
@@ -420,7 +467,7 @@ __device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, 
     else if constexpr (!EQ && T == 7 && MODE == 2)
         asm volatile(NBODY_LT7(NBODY_SYM_PAIR) NBODY_SYM_PAIR_M(7) NBODY_ROR_GEN NBODY_DIAG_OPS);
     else if constexpr (EQ && T == 0 && MODE == 0)
-        asm volatile(NBODY_LT0(NBODY_SYM_PAIR_EQ) NBODY_ROR_EQ NBODY_DIAG_OPS);
+        asm volatile("s_nop 1\n" NBODY_ROR_EQ NBODY_DIAG_OPS);
     else if constexpr (EQ && T == 0 && MODE == 1)
         asm volatile(NBODY_LT0(NBODY_SYM_PAIR_EQ) NBODY_SYM_PAIR_EQ(0) NBODY_ROR_EQ NBODY_DIAG_OPS);
     else if constexpr (EQ && T == 0 && MODE == 2)
@@ -579,11 +626,10 @@ __device__ __forceinline__ bool workgroup_all(bool ok)
 // the store; with m = 1 (or any power of two) the result is bit-identical to
 // the general weights.
 template <int A, int W, bool NEWTON, bool PIN = false>
-__device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, int npad, float eps2,
-                                         float* __restrict__ part, int P, int Q, float (*jacc)[64 * A * W])
+__device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, float eps2, const SlotMap& slots, int P,
+                                         int Q, float (*jacc)[64 * A * W])
 {
     constexpr int B = 64 * A * W;
-    const size_t plane = (size_t)(npad / B) * npad;
     const float m0 = pm[P * B].w;
 
     Lanes<A> me, acc;
@@ -612,18 +658,20 @@ __device__ __forceinline__ void sym_pair(const float4* __restrict__ pm, int n, i
     if (eq) sym_phases<A, W, NEWTON, true, PIN>(pm, n, eps2, Q, me, mi, acc, jacc);
     else sym_phases<A, W, NEWTON, false, PIN>(pm, n, eps2, Q, me, mi, acc, jacc);
     const float scale = eq ? m0 : 1.0f;
+    float* __restrict__ jdst = slots.j_side(P, Q, B);
     for (int k = threadIdx.x; k < B; k += 64 * W) {
-        const size_t o = (size_t)P * npad + (size_t)Q * B + k;
-        part[o] = jacc[0][k] * scale;
-        part[plane + o] = jacc[1][k] * scale;
-        part[2 * plane + o] = jacc[2][k] * scale;
+        jdst[k] = jacc[0][k] * scale;
+        jdst[slots.bplane + k] = jacc[1][k] * scale;
+        jdst[2 * slots.bplane + k] = jacc[2][k] * scale;
     }
+    float* __restrict__ idst = slots.i_side(P, Q, B);
+    const size_t iplane = slots.i_plane(P);
 #pragma unroll
     for (int u = 0; u < A; ++u) {
-        const size_t o = (size_t)Q * npad + (size_t)P * B + u * 64 * W + threadIdx.x;
-        part[o] = acc.x[u] * scale;
-        part[plane + o] = acc.y[u] * scale;
-        part[2 * plane + o] = acc.z[u] * scale;
+        const size_t o = u * 64 * W + threadIdx.x;
+        idst[o] = acc.x[u] * scale;
+        idst[iplane + o] = acc.y[u] * scale;
+        idst[2 * iplane + o] = acc.z[u] * scale;
     }
 }
 
@@ -700,8 +748,8 @@ __device__ __forceinline__ void sym_diag_block(const float4* __restrict__ pm, in
 // that no reduction reads. CONST_VIEW reads the j bodies through the constant
 // address space (pointers that come from memory rather than kernel arguments).
 template <int B, bool NEWTON, bool CONST_VIEW>
-__device__ __forceinline__ void sym_diag_piece(const float4* __restrict__ pm, int n, int npad, float eps2,
-                                               float* __restrict__ part, int P, int k, int threads)
+__device__ __forceinline__ void sym_diag_piece(const float4* __restrict__ pm, int n, float eps2, const SlotMap& slots,
+                                               int P, int k, int threads)
 {
     if (P * B + k * threads >= n) return;  // a piece made only of padding (workgroup-uniform)
     const int i = P * B + k * threads + (int)threadIdx.x;
@@ -714,11 +762,11 @@ __device__ __forceinline__ void sym_diag_piece(const float4* __restrict__ pm, in
     const int j0 = P * B;
     if constexpr (CONST_VIEW) sweep_const<1, NEWTON>(pm + j0, min(n - j0, B), eps2, me, acc);
     else sweep_sgpr<1, NEWTON>(pm + j0, min(n - j0, B), eps2, me, acc);
-    const size_t plane = (size_t)(npad / B) * npad;
-    const size_t o = (size_t)P * npad + i;
-    part[o] = acc.x[0];
-    part[plane + o] = acc.y[0];
-    part[2 * plane + o] = acc.z[0];
+    // slot P of block P's bodies: a band block, region B
+    float* __restrict__ dst = slots.j_side(P, P, B) + k * threads + threadIdx.x;
+    dst[0] = acc.x[0];
+    dst[slots.bplane] = acc.y[0];
+    dst[2 * slots.bplane] = acc.z[0];
 }
 
 }  // namespace nbody

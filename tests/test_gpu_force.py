@@ -143,6 +143,73 @@ def test_symmetric_equal_mass_blocks(oracle, shape):
         assert np.array_equal(fq[:, :3], 0.25 * f1[:, :3]), (shape, n)
 
 
+@pytest.mark.parametrize("shape", [(8, 512), (4, 256), (2, 128), (8, 2048)])
+def test_symmetric_column_bands(oracle, shape):
+    """A symmetric plan whose slots exceed slot_bytes runs in column bands of
+    blocks, carrying each body's running (sum, error) pair from band to band.
+    On a ragged N with unequal masses, for slot caps giving 2 bands up to one
+    column per band: forces against the FP64 oracle, run-to-run bit-equality,
+    and the cap reported by the plan."""
+    a, b = shape
+    n = 9 * b + 37
+    nb = -(-n // b)
+    pm, _ = bodies(n, seed=12)
+    d = dev(pm)
+    f64 = oracle.fp64_forces(pm, EPS2)
+    whole = 12 * b * nb * nb
+    for cap in (whole // 2, whole // 5, 12 * b * 3):
+        f = run_force(d, EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b, slot_bytes=cap)
+        mx, rms = force_errors(f, f64)
+        assert mx <= TOL_MAX and rms <= TOL_RMS, (shape, cap, mx, rms)
+        assert np.all(f[:, 3] == -7.0)
+        again = run_force(d, EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b, slot_bytes=cap)
+        assert np.array_equal(f.view(np.uint32), again.view(np.uint32)), (shape, cap)
+    o = _native.Options.make(variant=_native.SYMMETRIC, i_per_lane=a, jslice=b, slot_bytes=12345678)
+    plan = _native.Options()
+    _native.check(_native.lib().nbody_hip_plan(n, n, ctypes.byref(o), ctypes.byref(plan)), "plan")
+    assert plan.slot_bytes == 12345678
+
+
+@pytest.mark.parametrize("opt", [dict(variant=_native.SYMMETRIC, i_per_lane=4),
+                                 dict(variant=_native.SYMMETRIC, i_per_lane=8),
+                                 dict(variant=_native.SYMMETRIC, i_per_lane=4, slot_bytes=24 * 256 * 4),
+                                 dict(variant=_native.LDS, i_per_lane=1, jslice=1024),
+                                 dict(variant=_native.LDS, i_per_lane=2, jslice=5003),
+                                 dict(variant=_native.LDS, i_per_lane=4)],
+                         ids=["sym4", "sym8", "sym4-bands", "lds1-slices", "lds2-one-slice", "lds4"])
+def test_fp64_accumulation_plans(oracle, opt):
+    """compensated = 2: every term added in FP64 and the partial sums (block
+    slots, j slices) kept in FP64 until one rounding into the force. On a
+    ragged N with the reference's unequal masses: within the tolerance of
+    the FP64 oracle, deterministic, and reported by the plan."""
+    n = 5003
+    pm, _ = bodies(n, seed=13)
+    d = dev(pm)
+    f = run_force(d, EPS2, compensated=2, **opt)
+    mx, rms = force_errors(f, oracle.fp64_forces(pm, EPS2))
+    assert mx <= TOL_MAX and rms <= TOL_RMS, (opt, mx, rms)
+    assert np.all(f[:, 3] == -7.0)
+    again = run_force(d, EPS2, compensated=2, **opt)
+    assert np.array_equal(f.view(np.uint32), again.view(np.uint32))
+    o = _native.Options.make(compensated=2, **opt)
+    plan = _native.Options()
+    _native.check(_native.lib().nbody_hip_plan(n, n, ctypes.byref(o), ctypes.byref(plan)), "plan")
+    assert plan.compensated == 2 and plan.variant == opt["variant"]
+
+
+def test_fp64_accumulation_refusals():
+    pm, _ = bodies(3000, seed=14)
+    d = dev(pm)
+    f = force_buffer(3000)
+    for opt in (dict(variant=_native.SGPR), dict(variant=_native.DPP),
+                dict(variant=_native.SYMMETRIC, i_per_lane=2), dict(variant=_native.SYMMETRIC, i_per_lane=4,
+                                                                      jslice=512)):
+        o = _native.Options.make(compensated=2, **opt)
+        rc = _native.lib().nbody_hip_force(ptr(f), ptr(d), EPS2, 3000, 0, 3000, ctypes.byref(o), None)
+        assert rc != 0 and ("FP64" in _native.last_error()), (opt, _native.last_error())
+    assert np.all(np.isnan(host(f)[:, :3]))
+
+
 def test_symmetric_rejects_i_ranges():
     pm, _ = bodies(3000, seed=3)
     d = dev(pm)

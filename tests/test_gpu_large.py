@@ -1,6 +1,14 @@
-"""The N = 1048576 Plummer workload (eps^2 = 1e-4): plain against compensated
+"""Large systems.
+
+The N = 1048576 Plummer workload (eps^2 = 1e-4): plain against compensated
 accumulation, checked against the FP64 oracle on 4096 bodies (the 256
-nearest the centre, where |F| is smallest, and 3840 drawn at random).
+nearest the centre, where |F| is smallest, and 3840 drawn at random). Its
+default (symmetric) plan keeps 6.4 GB of block slots, so it runs in two
+column bands.
+
+N = 2^21 uniform ball: the default plan is the symmetric one (25.8 GB of
+block slots, kept a 4 GiB band at a time), checked against the FP64 oracle on
+2048 bodies the same way.
 
 Bounds: the compensated plans (default and LDS + TwoSum chunks) meet the
 1e-5 max / 1e-6 RMS target; one plain FP32 sum over 2^20 terms is reported
@@ -54,3 +62,22 @@ def test_plummer_1m_plain_vs_compensated(plummer_1m):
         assert mx <= 1e-5 and rms <= 1e-6, (name, mx, rms)
     assert errs["plain"][0] <= 1e-3
     assert errs["compensated"][1] < errs["plain"][1]
+
+
+def test_uniform_ball_2m_symmetric(oracle):
+    n = 1 << 21
+    pm, _ = ics.uniform_ball(n, seed=3)
+    plan = _native.Options()
+    _native.check(_native.lib().nbody_hip_plan(n, n, None, ctypes.byref(plan)), "plan")
+    assert plan.variant == _native.SYMMETRIC, plan.variant
+    r = np.linalg.norm(pm[:, :3], axis=1)
+    rng = np.random.default_rng(4)
+    idx = np.unique(np.concatenate([np.argsort(r)[:256], rng.choice(n, 1792, replace=False)]))
+    d_pm = dev(pm)
+    f = force_buffer(n)
+    _native.check(_native.lib().nbody_hip_force(ptr(f), ptr(d_pm), 0.01, n, 0, n, None, None), "force")
+    f = host(f)
+    mx, rms = force_errors(f[idx], oracle.fp64_forces(pm, 0.01, idx))
+    print(f"N=2^21 symmetric: max {mx:.2e} rms {rms:.2e}")
+    assert np.all(f[:, 3] == -7.0)
+    assert mx <= 1e-5 and rms <= 1e-6, (mx, rms)

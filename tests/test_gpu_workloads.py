@@ -3,9 +3,11 @@
 - N = 98304 uniform ball (eps^2 = 0.01, dt = 0.016, damping 0.95) stepped 20
   times on the device: every table entry's force on that final state against
   the FP64 oracle, on a sample that holds the 256 bodies of smallest |F| of
-  that state (the hardest for a relative bound) and 1792 random ones. The
-  reference recomputes its golden on the same positions every step
-  (src/nbody.cu:313-325,406-418); the same check is made at step 10 too.
+  that state (the hardest for a relative bound) and 1792 random ones, and on
+  every body against the library's GPU FP64 diagnostic (checked against the
+  oracle on the same sample). The reference recomputes its golden on the same
+  positions every step (src/nbody.cu:313-325,406-418); the same check is made
+  at step 10 too.
 - The 512-system ensemble (N log-uniform in [256, 16384], seed 4): each
   system's forces of steps 1 and 2 against the FP64 oracle on sampled bodies
   (its centre-most included), and three systems alone against the mix, bit
@@ -80,7 +82,8 @@ class _Sys:
 @pytest.fixture(scope="module")
 def c2_states(oracle):
     """The N = 98304 ball after 10 and after 20 device-resident steps, with a
-    hard sample of each state and the oracle's FP64 forces on it."""
+    hard sample of each state, the oracle's FP64 forces on it, and every
+    body's FP64 force from the GPU diagnostic."""
     n = 98304
     pm0, vm0 = ics.uniform_ball(n, seed=1)
     s = _Sys(pm0, vm0)
@@ -89,8 +92,13 @@ def c2_states(oracle):
         for upto in (10, 20):
             s.run(10)
             pm, _ = s.state()
-            idx = hard_sample(pm, fp64_all(pm), seed=upto)
-            states[upto] = (pm, idx, oracle.fp64_forces(pm, EPS2, idx))
+            every = fp64_all(pm)
+            idx = hard_sample(pm, every, seed=upto)
+            f64 = oracle.fp64_forces(pm, EPS2, idx)
+            # the diagnostic agrees with the oracle on the sample
+            diff = np.linalg.norm(every[idx] - f64, axis=1) / np.linalg.norm(f64, axis=1)
+            assert diff.max() < 1e-12
+            states[upto] = (pm, idx, f64, every)
     finally:
         s.close()
     return states
@@ -105,12 +113,43 @@ C2_ENTRIES = ["ComputeGravitation_HIP_Symmetric", "ComputeGravitation_HIP_LDS", 
 @pytest.mark.parametrize("step", [10, 20])
 def test_c2_evolved_state(c2_states, entry, step):
     from gpu_util import dev, run_entry
-    pm, idx, f64 = c2_states[step]
+    pm, idx, f64, every = c2_states[step]
     f, ms = run_entry(entry, dev(pm), EPS2)
     mx, rms = force_errors(f[idx], f64)
-    print(f"{entry} step {step}: {ms:.3f} ms, max {mx:.3e} rms {rms:.3e} on {idx.size} bodies")
+    amx, arms = force_errors(f, every)
+    print(f"{entry} step {step}: {ms:.3f} ms, oracle sample of {idx.size}: max {mx:.3e} rms {rms:.3e}; "
+          f"all {pm.shape[0]} bodies: max {amx:.3e} rms {arms:.3e}")
     assert np.all(f[:, 3] == -7.0)
     assert mx <= TOL_MAX and rms <= TOL_RMS, (entry, step, mx, rms)
+    assert amx <= TOL_MAX and arms <= TOL_RMS, (entry, step, amx, arms)
+
+
+@pytest.mark.slow
+def test_fp64_accumulation_on_a_blown_apart_state(oracle):
+    """N = 97792 uniform ball after 80 steps (dt = 0.016 is about one
+    dynamical time of the ball, so it has collapsed and blown apart): its
+    weakest-force bodies are the small remainder of large cancelling terms.
+    The symmetric plan with every term added in FP64 (compensated = 2) holds
+    the tolerance there; the default plan's error is printed beside it."""
+    from gpu_util import dev, run_force
+    n = 97792
+    pm0, vm0 = ics.uniform_ball(n, seed=1)
+    s = _Sys(pm0, vm0)
+    try:
+        s.run(80)
+        pm, _ = s.state()
+    finally:
+        s.close()
+    idx = hard_sample(pm, fp64_all(pm), seed=80)
+    f64 = oracle.fp64_forces(pm, EPS2, idx)
+    d = dev(pm)
+    acc = run_force(d, EPS2, variant=_native.SYMMETRIC, compensated=2)
+    default = run_force(d, EPS2)
+    mx, rms = force_errors(acc[idx], f64)
+    dmx, drms = force_errors(default[idx], f64)
+    print(f"N={n} after 80 steps: FP64-accumulated max {mx:.3e} rms {rms:.3e}; default max {dmx:.3e} "
+          f"rms {drms:.3e}")
+    assert mx <= TOL_MAX and rms <= TOL_RMS, (mx, rms)
 
 
 def _ensemble(systems):
@@ -135,11 +174,12 @@ def _ensemble_read(h, sizes, total):
 
 
 def _sample(pm, k, per=32):
-    """The centre-most body, the body of largest |x|, and random others."""
+    """The centre-most body and the body of largest |x| (the hardest for a
+    relative bound), then random others: (hard, random) index arrays."""
     n = pm.shape[0]
     r = np.linalg.norm(pm[:, :3], axis=1)
     rng = np.random.default_rng(1000 + k)
-    return np.unique(np.concatenate([[np.argmin(r), np.argmax(r)], rng.choice(n, min(n, per - 2), replace=False)]))
+    return np.unique([np.argmin(r), np.argmax(r)]), rng.choice(n, min(n, per - 2), replace=False)
 
 
 @pytest.mark.slow
@@ -158,14 +198,25 @@ def test_c4_ensemble_512_systems(oracle):
         pm2, vm2, f2 = _ensemble_read(h, sz, total)
     finally:
         L.nbody_hip_ensemble_destroy(h)
+    # every sampled body within the max bound; the RMS bound over the random
+    # bodies of all systems pooled (a few deliberately hard bodies in a
+    # 32-body sample would dominate a per-system RMS)
+    pooled = []
     for k, (pm0, _) in enumerate(systems):
         # step 1's force is the force of the initial state, step 2's of the state after step 1
         for state, f in ((pm0, f1[k]), (pm1[k], f2[k])):
-            idx = _sample(state, k)
-            mx, rms = force_errors(f[idx], oracle.fp64_forces(state, EPS2, idx))
-            worst = [max(worst[0], mx), max(worst[1], rms)]
-            assert mx <= TOL_MAX and rms <= TOL_RMS, (k, state.shape[0], mx, rms)
-    print(f"512 systems, {total} bodies: worst sampled max {worst[0]:.3e}, rms {worst[1]:.3e}")
+            hard, rand = _sample(state, k)
+            idx = np.concatenate([hard, rand])
+            f64 = oracle.fp64_forces(state, EPS2, idx)
+            rel = np.linalg.norm(f[idx, :3].astype(np.float64) - f64, axis=1) / np.linalg.norm(f64, axis=1)
+            worst[0] = max(worst[0], float(rel.max()))
+            assert rel.max() <= TOL_MAX, (k, state.shape[0], float(rel.max()))
+            pooled.append(rel[hard.size:])
+    pooled = np.concatenate(pooled)
+    worst[1] = float(np.sqrt(np.mean(pooled ** 2)))
+    print(f"512 systems, {total} bodies: worst sampled max {worst[0]:.3e}, pooled rms {worst[1]:.3e} "
+          f"over {pooled.size} random bodies")
+    assert worst[1] <= TOL_RMS
     order = np.argsort(sizes)
     for k in (int(order[0]), int(order[-1]), 137):
         ha, sa, ta = _ensemble([systems[k]])
@@ -244,7 +295,7 @@ def test_c5_served_trace(oracle):
     # the position by dt times that plus two roundings of x1.
     for k in (1, 5, 8, 10):
         pm0, vm0 = _request_ics(reqs[k])
-        idx = _sample(pm0, k, per=256)
+        idx = np.concatenate(_sample(pm0, k, per=256))
         f64 = oracle.fp64_forces(pm0, EPS2, idx)
         f = np.zeros((idx.size, 4), np.float32)
         f[:, :3] = f64
