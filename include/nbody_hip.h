@@ -112,6 +112,32 @@ float ComputeGravitation_HIP_Host(float *force, float *posMass, float softeningS
 int nbody_hip_force(float *force, const float *posMass, float softeningSquared, size_t N, size_t base,
                     size_t n, const nbody_hip_options *opt, void *stream);
 
+/* Bytes of the fixed device array that holds nbody_hip_force_noalloc's
+ * partial sums: one per GPU the library runs on, held until the process ends. */
+#define NBODY_HIP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. The same computation as nbody_hip_force, with the same
+ * arguments and kernels, but it allocates nothing: its partial sums live in
+ * a fixed array of NBODY_HIP_NOALLOC_SCRATCH_BYTES per GPU. Its plan is
+ * nbody_hip_force's when that plan's partial sums fit there. Otherwise the
+ * symmetric plan keeps its slots in narrower column bands (slot_bytes) and,
+ * unless opt fixes the shape (i_per_lane, jslice or waves_per_simd), takes
+ * larger blocks; with variant NBODY_HIP_AUTO a one-sided plan follows, with
+ * fewer, longer j slices, down to one slice, which keeps no partial sums. A
+ * fixed shape whose partial sums cannot fit is refused with an error and
+ * nothing is launched.
+ * Every call on a GPU shares that GPU's array, which keeps the last call's
+ * contents; the kernels write every partial sum they read, so nothing needs
+ * to start at zero. Calls that overlap on one GPU (on different streams, or
+ * graph replays that do) race on the array and give wrong results: order
+ * them. The first call on a device looks up the array's address
+ * (hipGetSymbolAddress) and creates the library's per-device events; make it
+ * before any graph capture. Returns 0 or an error code, as nbody_hip_force. */
+int nbody_hip_force_noalloc(float *force, const float *posMass, float softeningSquared, size_t N, size_t base,
+                            size_t n, const nbody_hip_options *opt, void *stream);
+
 /* integrateGravitation (src/nbody.cu:158-236) on device: for each body,
  * v += invMass * (F dt); v *= damping; x += v dt, with invMass = velInvMass.w.
  * posMass/velInvMass/force are device float4 arrays. Synchronous on the null

@@ -11,6 +11,9 @@ LIB_PATH = os.path.join(HERE, "lib", "libnbody_hip.so")
 
 ABI_VERSION = 3
 
+# bytes of the fixed device array of nbody_hip_force_noalloc (NBODY_HIP_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
 # ensemble modes (include/nbody_engine.h)
 BATCHED_GRAPH, BATCHED, PER_SYSTEM = 0, 1, 2
 
@@ -80,6 +83,7 @@ SIGNATURES = {
     "ComputeGravitation_HIP": (_f, [_p, _p, _f, _sz]),
     "ComputeGravitation_HIP_Host": (_f, [_p, _p, _f, _sz]),
     "nbody_hip_force": (_i, [_p, _p, _f, _sz, _sz, _sz, _OPT, _p]),
+    "nbody_hip_force_noalloc": (_i, [_p, _p, _f, _sz, _sz, _sz, _OPT, _p]),
     "integrateGravitation_HIP": (_f, [_p, _p, _p, _f, _f, _sz]),
     "nbody_hip_integrate": (_i, [_p, _p, _p, _f, _f, _sz, _p]),
     "nbody_hip_force_fp64": (_f, [_p, _p, _d, _sz]),

@@ -290,7 +290,87 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
 
 using namespace nbody::detail;
 
+// Partial sums of nbody_hip_force_noalloc: one fixed array per GPU, which the
+// device holds from the library's load to the end of the process. Host code
+// never passes this name to a kernel (on the host it is only a stand-in for
+// the symbol); it uses the address hipGetSymbolAddress returns for the
+// current device, kept in DeviceState::noalloc_scratch.
+static __device__ __align__(256) float4 scratch[NBODY_HIP_NOALLOC_SCRATCH_BYTES / sizeof(float4)];
+
 namespace {
+
+// This device's address of `scratch`, looked up once per device with that
+// device current.
+int noalloc_scratch(DeviceState& d, void** out)
+{
+    if (!d.noalloc_scratch) {
+        void* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        d.noalloc_scratch = ptr;
+    }
+    *out = d.noalloc_scratch;
+    return OK;
+}
+
+// A plan whose partial sums fit in `cap` bytes. The plan nbody_hip_force
+// would use when it fits. Otherwise a symmetric plan keeps its slots one
+// narrower column band at a time, then (unless the caller fixed the shape)
+// takes larger blocks, and for NBODY_HIP_AUTO the LDS plan after that; a
+// one-sided plan with no fixed shape gets fewer, longer j slices, down to a
+// single slice, which keeps no partial sums at all.
+int fit_plan(const DeviceState& d, size_t N, size_t n, const nbody_hip_options* opt, size_t cap, ForcePlan& p)
+{
+    const size_t need = plan_scratch_bytes(p, (int)n);
+    if (need <= cap) return OK;
+    const bool pinned = opt && (opt->i_per_lane > 0 || opt->jslice > 0 || opt->waves_per_simd > 0);
+    const int requested = opt ? opt->variant : NBODY_HIP_AUTO;
+    if (p.variant == VAR_SYMMETRIC) {
+        // narrower column bands (the running sums, 24 bytes per body, come
+        // out of the array too), then larger blocks unless the shape is fixed
+        ForcePlan q = p;
+        for (int block = p.jslice; block <= 64 * p.ipl * 4; block *= 2) {
+            if (!symmetric_shape_ok(q.ipl, block) || (pinned && block != p.jslice)) continue;
+            q.jslice = block;
+            q.nslices = (int)((N + block - 1) / block);
+            q.band_cap = 0;
+            if (plan_scratch_bytes(q, (int)n) <= cap) {
+                p = q;
+                return OK;
+            }
+            if (cap > 24 * n) {
+                q.band_cap = cap - 24 * n;
+                if (plan_scratch_bytes(q, (int)n) <= cap) {
+                    p = q;
+                    return OK;
+                }
+            }
+        }
+        if (pinned || requested != NBODY_HIP_AUTO) {
+            set_error("no symmetric block size keeps the partial sums of N = %zu within the %zu bytes of the "
+                      "fixed scratch array", N, cap);
+            return E_ARG;
+        }
+        nbody_hip_options o{VAR_LDS, 0, 0, opt ? opt->compensated : -1, opt ? opt->newton : 0, 0};
+        p = default_plan(VAR_LDS, N, n, d.cus, &o);
+        if (plan_scratch_bytes(p, (int)n) <= cap) return OK;
+    } else if (pinned) {
+        set_error("this plan keeps %zu bytes of partial sums, more than the %zu of the fixed scratch array "
+                  "(NBODY_HIP_NOALLOC_SCRATCH_BYTES)", need, cap);
+        return E_ARG;
+    }
+    const size_t per_slice = n * sizeof(float4) * (p.comp ? 2 : 1);
+    const size_t max_slices = cap / per_slice;
+    if (max_slices >= 2) {
+        p.jslice = round_up((int)((N + max_slices - 1) / max_slices), 256);
+        p.jslice = std::max(1, std::min(p.jslice, (int)N));
+        p.nslices = (int)((N + p.jslice - 1) / p.jslice);
+    }
+    if (plan_scratch_bytes(p, (int)n) > cap) {
+        p.jslice = (int)N;
+        p.nslices = 1;
+    }
+    return OK;
+}
 
 // Synchronous, event-timed evaluation on the null stream; returns ms or 0.
 float timed_force(const nbody_hip_options* opt, float* force, const float* pm, float eps2, size_t N)
@@ -422,6 +502,29 @@ int nbody_hip_force(float* force, const float* posMass, float softeningSquared, 
     ForcePlan plan;
     if (int r = make_plan(*d, N, n, opt, &plan)) return r;
     return run_force(*d, plan, force, posMass, softeningSquared, N, base, n, static_cast<hipStream_t>(stream));
+}
+
+int nbody_hip_force_noalloc(float* force, const float* posMass, float softeningSquared, size_t N, size_t base,
+                            size_t n, const nbody_hip_options* opt, void* stream)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (int r = check_args(force, posMass, N, base, n)) return r;
+    if (n == 0) return OK;
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    void* ptr = nullptr;
+    if (int r = noalloc_scratch(*d, &ptr)) return r;
+    ForcePlan plan;
+    if (int r = make_plan(*d, N, n, opt, &plan)) return r;
+    if (int r = fit_plan(*d, N, n, opt, NBODY_HIP_NOALLOC_SCRATCH_BYTES, plan)) return r;
+    if (int r = validate_plan(plan, N)) return r;
+    // the array as a buffer that is already as large as it can grow: run_force
+    // never allocates for it (the plan's partial sums fit, checked above)
+    Buffer fixed;
+    fixed.ptr = ptr;
+    fixed.cap = NBODY_HIP_NOALLOC_SCRATCH_BYTES;
+    return run_force(*d, plan, force, posMass, softeningSquared, N, base, n, static_cast<hipStream_t>(stream), &fixed,
+                     false);
 }
 
 int nbody_hip_integrate(float* posMass, float* velInvMass, const float* force, float dt, float damping, size_t N,

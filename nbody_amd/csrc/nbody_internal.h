@@ -99,6 +99,9 @@ struct DeviceState {
     bool partial_pending = false;
     Buffer partial64; // FP64 slice partials / energy partials
     Buffer stage_pm, stage_force, small;
+    // this device's copy of the no-allocation entry's fixed scratch array
+    // (its address from hipGetSymbolAddress, looked up on the first call)
+    void* noalloc_scratch = nullptr;
     PinnedBuffer host_stage;  // posMass in, force out, for the host-pointer entry
     // tuned plans: (requested variant, N, n, requested compensation, newton) -> plan
     std::map<std::tuple<int, size_t, size_t, int, int>, ForcePlan> tuned;
