@@ -36,7 +36,9 @@ in one run, src/nbody.cu:1024-1053):
                 CPU step of every ensemble system and of sampled requests.
 
 Single workloads on their own (--workload): ensemble, served, plummer, share
-(one GPU's i-range share of the N = 98304 system split over --shares GPUs).
+(one GPU's share of the N = 98304 system split over --shares GPUs: its i
+range against all N with the one-sided plans, and its share of the block
+pairs with the symmetric plan).
 """
 import argparse
 import ctypes
@@ -558,6 +560,45 @@ def bench_share(args):
         plan = _native.Options()
         _native.check(L.nbody_hip_plan(n, count, ctypes.byref(opt), ctypes.byref(plan)), "plan")
         results.append((call_ms, kms.value / nl.value, plan))
+    # the symmetric plan split by block columns instead: share k of g, every
+    # share's kernel time (the slowest one sets an 8-GPU step), FP64 partial
+    # forces summed over the shares against the FP64 direct sum
+    f64share = torch.zeros((n, 4), dtype=torch.float64, device=dev)
+
+    def pair_shares(sopt):
+        """Every share's call and kernel time with symmetric options sopt."""
+        calls, kernels = [], []
+        for k in range(g):
+            for _ in range(3):
+                _native.check(L.nbody_hip_force_pair_share(P(f64share), P(pm), eps2, n, k, g, ctypes.byref(sopt),
+                                                           None), "pair share")
+            torch.cuda.synchronize()
+            _native.check(L.nbody_hip_kernel_timer(1), "timer")
+            t0 = time.perf_counter()
+            for _ in range(launches):
+                _native.check(L.nbody_hip_force_pair_share(P(f64share), P(pm), eps2, n, k, g, ctypes.byref(sopt),
+                                                           None), "pair share")
+            torch.cuda.synchronize()
+            calls.append((time.perf_counter() - t0) * 1e3 / launches)
+            kms, nl = ctypes.c_double(), ctypes.c_int()
+            _native.check(L.nbody_hip_kernel_timer_read(ctypes.byref(kms), ctypes.byref(nl)), "timer read")
+            L.nbody_hip_kernel_timer(0)
+            kernels.append(kms.value / nl.value)
+        return calls, kernels
+
+    # the block shapes of the symmetric plan, the one whose slowest share is fastest kept
+    sweep = {}
+    for ipl, block in ((8, 512), (4, 256), (2, 128), (4, 512), (8, 1024)):
+        sopt = _native.Options.make(variant=_native.SYMMETRIC, i_per_lane=ipl, jslice=block)
+        sweep[(ipl, block)] = pair_shares(sopt)
+    best_shape = min(sweep, key=lambda s: max(sweep[s][1]))
+    share_ms, share_kernel_ms = sweep[best_shape]
+    sopt = _native.Options.make(variant=_native.SYMMETRIC, i_per_lane=best_shape[0], jslice=best_shape[1])
+    total = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    for k in range(g):
+        _native.check(L.nbody_hip_force_pair_share(P(f64share), P(pm), eps2, n, k, g, ctypes.byref(sopt), None),
+                      "pair share")
+        total += f64share[:, :3]
     ms, kernel_ms, plan = min(results, key=lambda r: r[0])
     opt = _native.Options.make(variant=plan.variant, i_per_lane=plan.i_per_lane, jslice=plan.jslice,
                                compensated=plan.compensated)
@@ -569,6 +610,21 @@ def bench_share(args):
     d = (force[:, :3].double() - ref).norm(dim=1) / ref.norm(dim=1)
     inter = float(count) * n
     tflops = 20.0 * inter / (kernel_ms * 1e-3) / 1e12
+    ds = (total.float().double() - f64[:, :3]).norm(dim=1) / f64[:, :3].norm(dim=1)
+    slowest = max(share_kernel_ms)
+    pair_share = {
+        "what": f"symmetric plan split over {g} GPUs by block columns (nbody_hip_force_pair_share): share k "
+                f"evaluates about 1/{g} of the block pairs for both bodies and writes FP64 partial forces of all "
+                f"N bodies, to be summed across GPUs by a reduce-scatter",
+        "shape": {"i_per_lane": best_shape[0], "block": best_shape[1]},
+        "slowest_kernel_ms_by_shape": {f"ipl{a}/B{b}": round(max(v[1]), 4) for (a, b), v in sweep.items()},
+        "ms_per_call": [round(t, 4) for t in share_ms],
+        "kernel_ms": [round(t, 4) for t in share_kernel_ms],
+        "slowest_kernel_ms": round(slowest, 4),
+        "credited_tflops_slowest": round(20.0 * float(n) * n / g / (slowest * 1e-3) / 1e12, 2),
+        "frac_slowest": round(20.0 * float(n) * n / g / (slowest * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+        "accuracy_of_the_sum": {"max_rel": float(ds.max()), "rms_rel": float(ds.pow(2).mean().sqrt())},
+    }
     return {
         "metric": METRIC, "value": round(inter / (ms * 1e-3) / 1e9, 2), "unit": "10^9 interactions/s",
         "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded uniform ball, unit masses)",
@@ -585,6 +641,7 @@ def bench_share(args):
                      "vs": "FP64 direct sum over the share's bodies"},
         "all_plans_ms_per_call": {f"{_native.VARIANT_NAMES[p.variant]}/ipl{p.i_per_lane}/js{p.jslice}": round(t, 4)
                                   for t, _, p in results},
+        "pair_share": pair_share,
     }
 
 

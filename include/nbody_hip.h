@@ -112,6 +112,23 @@ float ComputeGravitation_HIP_Host(float *force, float *posMass, float softeningS
 int nbody_hip_force(float *force, const float *posMass, float softeningSquared, size_t N, size_t base,
                     size_t n, const nbody_hip_options *opt, void *stream);
 
+/* One GPU's share of a whole system's block pairs, for running the symmetric
+ * plan on G GPUs. It has no counterpart in the reference, whose multi-GPU path
+ * splits the i range instead (src/nbody_multiGPU.cu:87-124; that split is
+ * nbody_hip_force with base, n). Share k of G evaluates the block pairs whose
+ * second block lies in the share's range of block columns, about 1/G of the
+ * work, for both bodies of each pair, and writes every body's partial force
+ * in FP64 to force64 (4 doubles per body; x, y, z written, w untouched). The
+ * G shares' partial forces add up to the force: across G GPUs, one
+ * reduce-scatter (sum, FP64) of them gives each GPU its own bodies' forces.
+ * FP64 keeps each share's running TwoSum value whole: a share's partial force
+ * on a body can be much larger than the net force. opt as for nbody_hip_force,
+ * with variant NBODY_HIP_AUTO or NBODY_HIP_SYMMETRIC. Asynchronous on stream,
+ * ordered with the library's other uses of its shared scratch as
+ * nbody_hip_force. Returns 0 or an error code. */
+int nbody_hip_force_pair_share(double *force64, const float *posMass, float softeningSquared, size_t N, int share,
+                               int shares, const nbody_hip_options *opt, void *stream);
+
 /* Bytes of the fixed device array that holds nbody_hip_force_noalloc's
  * partial sums: one per GPU the library runs on, held until the process ends. */
 #define NBODY_HIP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)

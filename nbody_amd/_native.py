@@ -84,6 +84,7 @@ SIGNATURES = {
     "ComputeGravitation_HIP_Host": (_f, [_p, _p, _f, _sz]),
     "nbody_hip_force": (_i, [_p, _p, _f, _sz, _sz, _sz, _OPT, _p]),
     "nbody_hip_force_noalloc": (_i, [_p, _p, _f, _sz, _sz, _sz, _OPT, _p]),
+    "nbody_hip_force_pair_share": (_i, [_p, _p, _f, _sz, _i, _i, _OPT, _p]),
     "integrateGravitation_HIP": (_f, [_p, _p, _p, _f, _f, _sz]),
     "nbody_hip_integrate": (_i, [_p, _p, _p, _f, _f, _sz, _p]),
     "nbody_hip_force_fp64": (_f, [_p, _p, _d, _sz]),

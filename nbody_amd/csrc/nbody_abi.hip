@@ -226,7 +226,7 @@ int check_args(const void* force, const void* pm, size_t N, size_t base, size_t 
 
 // Queue one force evaluation on `stream` (caller holds g_mutex).
 int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
-              size_t n, hipStream_t stream, Buffer* own, bool timed)
+              size_t n, hipStream_t stream, Buffer* own, bool timed, double* force64)
 {
     Buffer& buf = own ? *own : d.partial;
     if (plan.variant == VAR_SYMMETRIC && (base != 0 || n != N)) {
@@ -242,6 +242,7 @@ int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* 
     p.jslice = plan.jslice;
     p.nslices = plan.nslices;
     p.force = force;
+    p.force64 = force64;
     const size_t bytes = plan_scratch_bytes(plan, (int)n);
     const bool shared = bytes && !own;
     if (shared) {
@@ -502,6 +503,33 @@ int nbody_hip_force(float* force, const float* posMass, float softeningSquared, 
     ForcePlan plan;
     if (int r = make_plan(*d, N, n, opt, &plan)) return r;
     return run_force(*d, plan, force, posMass, softeningSquared, N, base, n, static_cast<hipStream_t>(stream));
+}
+
+int nbody_hip_force_pair_share(double* force64, const float* posMass, float softeningSquared, size_t N, int share,
+                               int shares, const nbody_hip_options* opt, void* stream)
+{
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (int r = check_args(force64, posMass, N, 0, N)) return r;
+    if (shares < 1 || share < 0 || share >= shares) {
+        set_error("share %d of %d: need 0 <= share < shares", share, shares);
+        return E_ARG;
+    }
+    const int requested = opt ? opt->variant : NBODY_HIP_AUTO;
+    if (requested != NBODY_HIP_AUTO && requested != NBODY_HIP_SYMMETRIC) {
+        set_error("a share of the block pairs needs the symmetric plan (variant %d given)", requested);
+        return E_ARG;
+    }
+    DeviceState* d;
+    if (int r = device_state(&d)) return r;
+    nbody_hip_options o{};
+    if (opt) o = *opt;
+    else o.compensated = -1;
+    o.variant = NBODY_HIP_SYMMETRIC;
+    ForcePlan plan;
+    if (int r = make_plan(*d, N, N, &o, &plan)) return r;
+    symmetric_share_columns(plan.nslices, share, shares, &plan.col0, &plan.col1);
+    return run_force(*d, plan, nullptr, posMass, softeningSquared, N, 0, N, static_cast<hipStream_t>(stream), nullptr,
+                     true, force64);
 }
 
 int nbody_hip_force_noalloc(float* force, const float* posMass, float softeningSquared, size_t N, size_t base,

@@ -133,9 +133,10 @@ int check_args(const void* force, const void* pm, size_t N, size_t base, size_t 
 // shared buffer are ordered across streams (each waits for the previous
 // one's kernels), and refused on a stream that is capturing a graph, since a
 // later, larger call may reallocate the buffer under the graph. The kernel
-// timer records only when `timed` is set.
+// timer records only when `timed` is set. With force64 (a symmetric plan over
+// a share of the columns) the partial forces go there in FP64 instead.
 int run_force(DeviceState& d, const ForcePlan& plan, float* force, const float* pm, float eps2, size_t N, size_t base,
-              size_t n, hipStream_t stream, Buffer* own = nullptr, bool timed = true);
+              size_t n, hipStream_t stream, Buffer* own = nullptr, bool timed = true, double* force64 = nullptr);
 
 }  // namespace detail
 }  // namespace nbody

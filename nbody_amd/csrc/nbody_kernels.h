@@ -32,6 +32,7 @@ struct ForcePlan {
     bool newton = false;
     int occ = 0;        // symmetric: waves per SIMD cap (0 = register-limited occupancy)
     size_t band_cap = 0;  // symmetric: bytes of slot partials one band may keep (0 = kSymBandCap)
+    int col0 = 0, col1 = 0;  // symmetric: the column blocks [col0, col1) of the block triangle (0, 0 = all)
 };
 
 // Bytes of block-slot partials one launch of the symmetric plan keeps by
@@ -41,7 +42,12 @@ constexpr size_t kSymBandCap = size_t(4) << 30;
 // The column bands [q0, q1) of a symmetric plan with nb blocks of B bodies
 // whose partials (elem bytes each) fit in cap bytes each (a band of one
 // column always runs).
-std::vector<std::pair<int, int>> symmetric_bands(int nb, int B, size_t cap, int elem = 4);
+std::vector<std::pair<int, int>> symmetric_bands(int nb, int B, size_t cap, int elem = 4, int c0 = 0, int c1 = -1);
+
+// The column blocks [c0, c1) of share k of G of a symmetric plan with nb
+// blocks: consecutive column ranges of about equal work (column Q holds Q
+// block pairs and one diagonal block).
+void symmetric_share_columns(int nb, int k, int G, int* c0, int* c1);
 
 struct ForceParams {
     const float4* pm = nullptr;  // posMass (x, y, z, m) of the whole system: j source and i source
@@ -54,6 +60,8 @@ struct ForceParams {
     float4* hi = nullptr;        // slice partials [nslices][ni]; nullptr -> write force directly
     float4* lo = nullptr;        // compensation partials [nslices][ni] (comp only)
     double* hi64 = nullptr;      // FP64 partials (acc64): slices [nslices][3][ni], or the symmetric slot planes
+    double* force64 = nullptr;   // symmetric column share: FP64 partial forces, 4 doubles per body (w untouched),
+                                 // written instead of force
     float* force = nullptr;      // 4 floats per i body; x, y, z written, w left alone
 };
 

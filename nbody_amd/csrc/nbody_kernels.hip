@@ -309,17 +309,19 @@ __global__ __launch_bounds__(64 * W) void k_force_sym(SymParams p)
 // (sum, error) pair of each body carries from band to band in `state`
 // ([2][3][n]; the first band starts it, a body of a block past the band adds
 // nothing yet) and the last band rounds it into the force; each body's slots
-// are then folded in band order, the same fixed order on every run.
-__global__ __launch_bounds__(kBlock) void k_reduce_band(SlotMap m, int n, int B, int q0, int q1, int nb,
-                                                        float* __restrict__ state, float* __restrict__ force)
+// are then folded in band order, the same fixed order on every run. With
+// force64 the last band writes the pair's value in FP64 instead (a share of
+// the columns, whose partial force is summed with the other shares').
+__global__ __launch_bounds__(kBlock) void k_reduce_band(SlotMap m, int n, int B, int q0, int q1, bool first,
+                                                        bool last, float* __restrict__ state,
+                                                        float* __restrict__ force, double* __restrict__ force64)
 {
     constexpr int G = 8;
     const int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= n) return;
     const int c = blockIdx.y;
-    const bool first = q0 == 0, last = q1 == nb;
     const int blk = b / B;
-    if (!first && blk >= q1) return;
+    if (!first && !last && blk >= q1) return;
     const float* src = nullptr;
     int slots = 0;
     size_t row = 0;
@@ -354,7 +356,9 @@ __global__ __launch_bounds__(kBlock) void k_reduce_band(SlotMap m, int n, int B,
         s = two_sum(s, src[(size_t)k * row], err);
         e += err;
     }
-    if (last) {
+    if (last && force64) {
+        force64[4 * (size_t)b + c] = (double)s + (double)e;
+    } else if (last) {
         force[4 * (size_t)b + c] = s + e;
     } else {
         state[(size_t)c * n + b] = s;
@@ -390,16 +394,16 @@ __global__ __launch_bounds__(64) void k_force_sym64(SymParams64 p)
 // k_reduce_band for FP64 slots: each body's slots of the band added in slot
 // order in FP64, the running sum carried between bands in `state` ([3][n]
 // doubles), rounded once into the force by the last band.
-__global__ __launch_bounds__(kBlock) void k_reduce_band64(SlotMap64 m, int n, int B, int q0, int q1, int nb,
-                                                          double* __restrict__ state, float* __restrict__ force)
+__global__ __launch_bounds__(kBlock) void k_reduce_band64(SlotMap64 m, int n, int B, int q0, int q1, bool first,
+                                                          bool last, double* __restrict__ state,
+                                                          float* __restrict__ force, double* __restrict__ force64)
 {
     constexpr int G = 8;
     const int b = blockIdx.x * kBlock + threadIdx.x;
     if (b >= n) return;
     const int c = blockIdx.y;
-    const bool first = q0 == 0, last = q1 == nb;
     const int blk = b / B;
-    if (!first && blk >= q1) return;
+    if (!first && !last && blk >= q1) return;
     const double* src = nullptr;
     int slots = 0;
     size_t row = 0;
@@ -422,7 +426,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce_band64(SlotMap64 m, int n, in
         for (int g = 0; g < G; ++g) s += v[g];
     }
     for (; k < slots; ++k) s += src[(size_t)k * row];
-    if (last) force[4 * (size_t)b + c] = (float)s;
+    if (last && force64) force64[4 * (size_t)b + c] = s;
+    else if (last) force[4 * (size_t)b + c] = (float)s;
     else state[(size_t)c * n + b] = s;
 }
 
@@ -693,6 +698,22 @@ hipError_t go_sym_shape(const ForcePlan& plan, const SymParams& sp, int band_blo
 // Floats of slot regions band [q0, q1) keeps: region A then region B.
 size_t band_floats(int q0, int q1, int B) { return (size_t)3 * (q1 - q0) * B * ((size_t)q0 + q1); }
 
+// Where band [q0, q1) keeps its slots, from `base`: region A, then region B.
+template <class T>
+SlotMapT<T> band_slots(T* base, int q0, int q1, int B)
+{
+    const int w = q1 - q0;
+    SlotMapT<T> m;
+    m.a = base;
+    m.arow = q0 * B;
+    m.aplane = (size_t)w * q0 * B;
+    m.b = base + 3 * m.aplane;
+    m.brow = w * B;
+    m.bplane = (size_t)q1 * w * B;
+    m.q0 = q0;
+    return m;
+}
+
 template <int A, int W>
 int natural_occupancy(bool newton)
 {
@@ -721,14 +742,32 @@ int symmetric_natural_occupancy(const ForcePlan& plan)
                                                                   : natural_occupancy<8, 4>(plan.newton);
 }
 
-std::vector<std::pair<int, int>> symmetric_bands(int nb, int B, size_t cap, int elem)
+void symmetric_share_columns(int nb, int k, int G, int* c0, int* c1)
+{
+    // work of the columns below c: sum of (Q + 1) over Q < c = c (c + 1) / 2
+    const double total = 0.5 * nb * (nb + 1.0);
+    auto boundary = [&](int j) {
+        if (j <= 0) return 0;
+        if (j >= G) return nb;
+        const double want = total * j / G;
+        int c = (int)std::floor(std::sqrt(2.0 * want));
+        while (c > 0 && 0.5 * c * (c + 1.0) > want) --c;
+        while (c < nb && 0.5 * c * (c + 1.0) < want) ++c;
+        return c;
+    };
+    *c0 = boundary(k);
+    *c1 = boundary(k + 1);
+}
+
+std::vector<std::pair<int, int>> symmetric_bands(int nb, int B, size_t cap, int elem, int c0, int c1)
 {
     std::vector<std::pair<int, int>> bands;
+    if (c1 < 0) c1 = nb;
     const double per_col = 3.0 * elem * B;  // bytes per band column and block below q1, both regions
-    for (int q0 = 0; q0 < nb;) {
+    for (int q0 = c0; q0 < c1;) {
         // the widest band [q0, q0 + w) whose regions, 3 elem B w (2 q0 + w) bytes, fit the cap
         const double w = -(double)q0 + std::sqrt((double)q0 * q0 + (double)cap / per_col);
-        int width = std::max(1, std::min(nb - q0, (int)w));
+        int width = std::max(1, std::min(c1 - q0, (int)w));
         while (width > 1 && band_floats(q0, q0 + width, B) * elem > cap) --width;
         bands.emplace_back(q0, q0 + width);
         q0 += width;
@@ -740,8 +779,8 @@ size_t plan_scratch_bytes(const ForcePlan& plan, int ni)
 {
     if (plan.variant == VAR_SYMMETRIC) {
         const int elem = plan.acc64 ? 8 : 4;
-        const auto bands =
-            symmetric_bands(plan.nslices, plan.jslice, plan.band_cap ? plan.band_cap : kSymBandCap, elem);
+        const auto bands = symmetric_bands(plan.nslices, plan.jslice, plan.band_cap ? plan.band_cap : kSymBandCap,
+                                           elem, plan.col0, plan.col1 > 0 ? plan.col1 : plan.nslices);
         size_t most = 0;
         for (const auto& b : bands) most = std::max(most, band_floats(b.first, b.second, plan.jslice));
         // the running sums carried between bands: (sum, error) float pairs or one double per component
@@ -772,78 +811,66 @@ hipError_t launch_force(const ForcePlan& plan, const ForceParams& p, hipStream_t
         if (e == hipSuccess && after) e = hipEventRecord(after, stream);
         return e;
     }
-    if (plan.variant == VAR_SYMMETRIC && plan.acc64) {
-        if (p.i0 != 0 || p.ni != p.n || plan.jslice != 64 * plan.ipl || (plan.ipl != 4 && plan.ipl != 8) ||
-            p.hi64 == nullptr)
-            return hipErrorInvalidValue;
-        const int B = plan.jslice, nb = plan.nslices;
-        const auto bands = symmetric_bands(nb, B, plan.band_cap ? plan.band_cap : kSymBandCap, 8);
-        double* base = p.hi64;
-        size_t most = 0;
-        for (const auto& b : bands) most = std::max(most, band_floats(b.first, b.second, B));
-        double* state = bands.size() > 1 ? base + most : nullptr;
-        for (size_t k = 0; k < bands.size(); ++k) {
-            const int q0 = bands[k].first, q1 = bands[k].second, w = q1 - q0;
-            SymParams64 sp;
-            sp.pm = p.pm;
-            sp.n = p.n;
-            sp.q0 = q0;
-            sp.npairs = (int)(((long long)q1 * (q1 - 1) - (long long)q0 * (q0 - 1)) / 2);
-            sp.eps2 = p.eps2;
-            sp.slots.a = base;
-            sp.slots.arow = q0 * B;
-            sp.slots.aplane = (size_t)w * q0 * B;
-            sp.slots.b = base + 3 * sp.slots.aplane;
-            sp.slots.brow = w * B;
-            sp.slots.bplane = (size_t)q1 * w * B;
-            sp.slots.q0 = q0;
-            const int tasks = sp.npairs + w * plan.ipl;
-            if (plan.ipl == 4) {
-                if (plan.newton) k_force_sym64<4, true><<<tasks, 64, 0, stream>>>(sp);
-                else k_force_sym64<4, false><<<tasks, 64, 0, stream>>>(sp);
-            } else {
-                if (plan.newton) k_force_sym64<8, true><<<tasks, 64, 0, stream>>>(sp);
-                else k_force_sym64<8, false><<<tasks, 64, 0, stream>>>(sp);
-            }
-            e = hipGetLastError();
-            if (e == hipSuccess && after && k + 1 == bands.size()) e = hipEventRecord(after, stream);
-            if (e != hipSuccess) return e;
-            k_reduce_band64<<<dim3((p.n + kBlock - 1) / kBlock, 3), kBlock, 0, stream>>>(sp.slots, p.n, B, q0, q1,
-                                                                                        nb, state, p.force);
-            e = hipGetLastError();
-            if (e != hipSuccess) return e;
-        }
-        return hipSuccess;
-    }
     if (plan.variant == VAR_SYMMETRIC) {
-        if (p.i0 != 0 || p.ni != p.n || !symmetric_shape_ok(plan.ipl, plan.jslice) || p.hi == nullptr)
-            return hipErrorInvalidValue;
+        const bool ok_shape = plan.acc64 ? plan.jslice == 64 * plan.ipl && (plan.ipl == 4 || plan.ipl == 8)
+                                         : symmetric_shape_ok(plan.ipl, plan.jslice);
         const int B = plan.jslice, nb = plan.nslices;
-        const auto bands = symmetric_bands(nb, B, plan.band_cap ? plan.band_cap : kSymBandCap);
-        float* base = reinterpret_cast<float*>(p.hi);
+        const int c0 = plan.col0, c1 = plan.col1 > 0 ? plan.col1 : nb;
+        // slots are needed unless the column range is empty (a share with no columns)
+        const bool no_slots = plan.acc64 ? p.hi64 == nullptr : p.hi == nullptr;
+        if (p.i0 != 0 || p.ni != p.n || !ok_shape || (no_slots && c1 > c0)) return hipErrorInvalidValue;
+        const size_t cap = plan.band_cap ? plan.band_cap : kSymBandCap;
+        auto bands = symmetric_bands(nb, B, cap, plan.acc64 ? 8 : 4, c0, c1);
         size_t most = 0;
         for (const auto& b : bands) most = std::max(most, band_floats(b.first, b.second, B));
-        float* state = bands.size() > 1 ? base + most : nullptr;
+        // a share with no columns still writes its (zero) partial forces
+        if (bands.empty()) bands.emplace_back(c0, c0);
+        const dim3 gr((p.n + kBlock - 1) / kBlock, 3);
+        e = hipSuccess;
         for (size_t k = 0; k < bands.size(); ++k) {
             const int q0 = bands[k].first, q1 = bands[k].second, w = q1 - q0;
-            SymParams sp;
-            sp.pm = p.pm;
-            sp.n = p.n;
-            sp.q0 = q0;
-            sp.npairs = (int)(((long long)q1 * (q1 - 1) - (long long)q0 * (q0 - 1)) / 2);
-            sp.eps2 = p.eps2;
-            sp.slots.a = base;
-            sp.slots.arow = q0 * B;
-            sp.slots.aplane = (size_t)w * q0 * B;
-            sp.slots.b = base + 3 * sp.slots.aplane;
-            sp.slots.brow = w * B;
-            sp.slots.bplane = (size_t)q1 * w * B;
-            sp.slots.q0 = q0;
-            e = go_sym_shape(plan, sp, w, stream);
-            if (e == hipSuccess && after && k + 1 == bands.size()) e = hipEventRecord(after, stream);
-            if (e != hipSuccess) return e;
-            k_reduce_band<<<dim3((p.n + kBlock - 1) / kBlock, 3), kBlock, 0, stream>>>(sp.slots, p.n, B, q0, q1, nb,
-                                                                                      state, p.force);
+            const bool first = k == 0, last = k + 1 == bands.size();
+            const int npairs = (int)(((long long)q1 * (q1 - 1) - (long long)q0 * (q0 - 1)) / 2);
+            if (plan.acc64) {
+                SymParams64 sp;
+                sp.pm = p.pm;
+                sp.n = p.n;
+                sp.q0 = q0;
+                sp.npairs = npairs;
+                sp.eps2 = p.eps2;
+                sp.slots = band_slots(p.hi64, q0, q1, B);
+                const int tasks = npairs + w * plan.ipl;
+                if (tasks > 0) {
+                    if (plan.ipl == 4) {
+                        if (plan.newton) k_force_sym64<4, true><<<tasks, 64, 0, stream>>>(sp);
+                        else k_force_sym64<4, false><<<tasks, 64, 0, stream>>>(sp);
+                    } else {
+                        if (plan.newton) k_force_sym64<8, true><<<tasks, 64, 0, stream>>>(sp);
+                        else k_force_sym64<8, false><<<tasks, 64, 0, stream>>>(sp);
+                    }
+                    e = hipGetLastError();
+                }
+                if (e == hipSuccess && after && last) e = hipEventRecord(after, stream);
+                if (e != hipSuccess) return e;
+                double* state = bands.size() > 1 ? p.hi64 + most : nullptr;
+                k_reduce_band64<<<gr, kBlock, 0, stream>>>(sp.slots, p.n, B, q0, q1, first, last, state, p.force,
+                                                           p.force64);
+            } else {
+                SymParams sp;
+                sp.pm = p.pm;
+                sp.n = p.n;
+                sp.q0 = q0;
+                sp.npairs = npairs;
+                sp.eps2 = p.eps2;
+                float* base = reinterpret_cast<float*>(p.hi);
+                sp.slots = band_slots(base, q0, q1, B);
+                if (npairs + w > 0) e = go_sym_shape(plan, sp, w, stream);
+                if (e == hipSuccess && after && last) e = hipEventRecord(after, stream);
+                if (e != hipSuccess) return e;
+                float* state = bands.size() > 1 ? base + most : nullptr;
+                k_reduce_band<<<gr, kBlock, 0, stream>>>(sp.slots, p.n, B, q0, q1, first, last, state, p.force,
+                                                         p.force64);
+            }
             e = hipGetLastError();
             if (e != hipSuccess) return e;
         }

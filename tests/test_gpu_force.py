@@ -210,6 +210,55 @@ def test_fp64_accumulation_refusals():
     assert np.all(np.isnan(host(f)[:, :3]))
 
 
+def pair_shares(d_pm, n, shares, **opt):
+    """Every share's FP64 partial forces and their sum."""
+    L = _native.lib()
+    o = _native.Options.make(**opt) if opt else None
+    parts = []
+    for k in range(shares):
+        out = dev(np.full((n, 4), -9.0, np.float64))
+        rc = L.nbody_hip_force_pair_share(ptr(out), ptr(d_pm), EPS2, n, k, shares, ctypes.byref(o) if o else None,
+                                          None)
+        _native.check(rc, "nbody_hip_force_pair_share")
+        parts.append(host(out))
+    return parts, sum(p[:, :3] for p in parts)
+
+
+@pytest.mark.parametrize("shares", [1, 3, 8, 13])
+@pytest.mark.parametrize("opt", [{}, dict(i_per_lane=4, jslice=256, slot_bytes=12 * 256 * 6),
+                                 dict(compensated=2)], ids=["default", "bands", "fp64"])
+def test_pair_shares_add_up(oracle, shares, opt):
+    """The block pairs split over G GPUs by columns: the G shares' FP64
+    partial forces add up to the force (within the path's tolerance of the
+    FP64 oracle), each share leaves w alone, a share past the last column
+    writes zeros, and one share is the whole symmetric evaluation bit for
+    bit."""
+    n = 9 * 512 + 37
+    pm, _ = bodies(n, seed=20)
+    d = dev(pm)
+    parts, total = pair_shares(d, n, shares, **opt)
+    for p in parts:
+        assert np.all(p[:, 3] == -9.0)
+    mx, rms = force_errors(total, oracle.fp64_forces(pm, EPS2))
+    assert mx <= TOL_MAX and rms <= TOL_RMS, (shares, opt, mx, rms)
+    if shares == 1:
+        f = run_force(d, EPS2, variant=_native.SYMMETRIC, **opt)
+        assert np.array_equal(total.astype(np.float32).view(np.uint32), f[:, :3].view(np.uint32))
+
+
+def test_pair_share_refusals():
+    pm, _ = bodies(3000, seed=21)
+    d = dev(pm)
+    out = dev(np.full((3000, 4), -9.0, np.float64))
+    L = _native.lib()
+    assert L.nbody_hip_force_pair_share(ptr(out), ptr(d), EPS2, 3000, 3, 3, None, None) != 0
+    assert "share" in _native.last_error()
+    o = _native.Options.make(variant=_native.LDS)
+    assert L.nbody_hip_force_pair_share(ptr(out), ptr(d), EPS2, 3000, 0, 2, ctypes.byref(o), None) != 0
+    assert "symmetric" in _native.last_error()
+    assert np.all(host(out) == -9.0)
+
+
 def test_symmetric_rejects_i_ranges():
     pm, _ = bodies(3000, seed=3)
     d = dev(pm)
