@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kWave) void k_force_batch(const ForceTask* __restri
 // 3 x kBodyTaskBodies threads per task: the threads of component c add that
 // component of the task's bodies over their slots (eight slots in flight,
 // each slot row read as one contiguous 1 KiB run), then the first
-// kBodyTaskBodies threads integrate. The slot sum is k_reduce_slots' (slot
+// kBodyTaskBodies threads integrate. The slot sum is k_reduce_band's (slot
 // order, TwoSum, rounded once) and the update is k_integrate's, both in
 // nbody_kernels.hip.
 __global__ __launch_bounds__(3 * kBodyTaskBodies) void k_reduce_integrate_batch(const BodyTask* __restrict__ tasks,
