@@ -86,7 +86,7 @@ def build(verbose=False):
     # the command-line driver (host code only), linked against the library
     os.makedirs(BINDIR, exist_ok=True)
     exe = os.path.join(BINDIR, "nbody")
-    _run([cc, "-O2", "-std=c++17", "-Wall", "-Wno-unused-command-line-argument", "-I", INCLUDE,
+    _run([cc, "-O2", "-std=c++17", "-pthread", "-Wall", "-Wno-unused-command-line-argument", "-I", INCLUDE,
           os.path.join(CSRC, "nbody_main.cpp"), "-o", exe + ".tmp", "-L", LIBDIR, "-lnbody_hip",
           "-Wl,-rpath,$ORIGIN/../lib"])
     os.replace(exe + ".tmp", exe)

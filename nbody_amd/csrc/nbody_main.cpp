@@ -3,9 +3,9 @@
 // (src/nbody.cu:735-1122), so its runs and `scripts/nbody-stats.py` parsing
 // carry over. Differences: every algorithm runs on the GPU (the CPU table
 // rows are the reference's, not this library's); the crosscheck golden is the
-// library's FP64 direct sum instead of CPU_SOA; positions and velocities stay
-// on the device and are integrated there; there is no keyboard control or
-// OpenGL window.
+// library's FP64 direct sum unless --golden=cpu_soa asks for the reference's
+// CPU_SOA sum; positions and velocities stay on the device and are integrated
+// there; there is no keyboard control or OpenGL window.
 #include "nbody_hip.h"
 #include "nbody_ics.h"
 
@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -59,6 +60,7 @@ int g_maxGPUs = 0;
 int g_numGPUs = 0;
 int g_bNoCPU = 0;
 int g_bCrossCheck = 1;
+int g_bGoldenSOA = 0;  // --golden=cpu_soa
 
 int maxAlgorithmIdx()
 {
@@ -89,7 +91,9 @@ void print_usage()
     fprintf(stderr, "    --gpus=<N> | -g <N>\n        GPUs to use (this build computes on one).\n"
                     "        [default: number of available GPUs]\n\n");
     fprintf(stderr, "    --no-cpu\n        Disables the host-pointer algorithm and the crosscheck.\n\n");
-    fprintf(stderr, "    --no-crosscheck\n        Disables cross-validation against the FP64 direct sum.\n\n");
+    fprintf(stderr, "    --no-crosscheck\n        Disables cross-validation against the golden forces.\n\n");
+    fprintf(stderr, "    --golden=<fp64|cpu_soa>\n        Crosscheck golden: the FP64 direct sum on the GPU, or the\n"
+                    "        reference's CPU_SOA sum on the host's threads. [default: fp64]\n\n");
     fprintf(stderr, "    --iterations=<N> | -i <N>\n        Passes through the algorithm list. [default: loop forever]\n\n");
     fprintf(stderr, "    --cycle-after=<N> | -c <N>\n        Steps before moving to the next algorithm.\n"
                     "        [default: none, don't cycle]\n\n");
@@ -127,6 +131,53 @@ float relError(float a, float b)
     return fabsf(a - b) / b;
 }
 
+// The golden of src/nbody.cu:311-321 with --golden=cpu_soa: CPU_SOA's FP32
+// sum (src/nbody_CPU_SOA.cpp:68-96, the pair term of
+// src/bodybodyInteraction.cuh:40-65 with rsqrtf as 1/sqrtf, src/chCUDA.h:143-146)
+// over host positions. Bodies i are split over the host's threads; j runs in
+// 16 interleaved partial sums added at the end, the shape of a 16-lane
+// `omp simd` reduction, so the result equals the reference's build up to the
+// order in which its reduction adds those lanes.
+void golden_cpu_soa(const float* pm, float eps2, size_t n, float* golden)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nthreads = std::min<size_t>(hw, (n + 255) / 256);
+    auto rows = [&](size_t t) {
+        for (size_t i = t; i < n; i += nthreads) {
+            const float xi = pm[4 * i + 0], yi = pm[4 * i + 1], zi = pm[4 * i + 2];
+            float ax[16] = {}, ay[16] = {}, az[16] = {};
+            for (size_t j0 = 0; j0 < n; j0 += 16) {
+                const size_t m = std::min<size_t>(16, n - j0);
+                for (size_t l = 0; l < m; l++) {
+                    const float* b = pm + 4 * (j0 + l);
+                    const float dx = b[0] - xi, dy = b[1] - yi, dz = b[2] - zi;
+                    float d2 = dx * dx + dy * dy + dz * dz;
+                    d2 += eps2;
+                    const float inv = 1.0f / sqrtf(d2);
+                    const float s = b[3] * (inv * inv * inv);
+                    ax[l] += dx * s;
+                    ay[l] += dy * s;
+                    az[l] += dz * s;
+                }
+            }
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            for (int l = 0; l < 16; l++) {
+                sx += ax[l];
+                sy += ay[l];
+                sz += az[l];
+            }
+            golden[4 * i + 0] = sx;
+            golden[4 * i + 1] = sy;
+            golden[4 * i + 2] = sz;
+            golden[4 * i + 3] = 0.0f;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nthreads; t++) pool.emplace_back(rows, t);
+    rows(0);
+    for (auto& th : pool) th.join();
+}
+
 // One step of src/nbody.cu:293-470: golden, force, crosscheck, integrate.
 // Returns nonzero when the algorithm was skipped or failed.
 int ComputeGravitation(State& s, float* ms, float* maxRelError, const AlgoDef& alg, int crossCheck)
@@ -135,7 +186,10 @@ int ComputeGravitation(State& s, float* ms, float* maxRelError, const AlgoDef& a
     if (g_bNoCPU && alg.type == ALGO_AOS) return 1;
     const size_t n = s.n;
     const float eps2 = kSoftening * kSoftening;
-    if (crossCheck) {
+    if (crossCheck && g_bGoldenSOA) {
+        HIP_OK(hipMemcpy(s.h_pm.data(), s.d_pm, 4 * n * sizeof(float), hipMemcpyDeviceToHost));
+        golden_cpu_soa(s.h_pm.data(), eps2, n, s.golden.data());
+    } else if (crossCheck) {
         if (nbody_hip_force_fp64(s.d_golden, s.d_pm, (double)eps2, n) <= 0.0f) {
             fprintf(stderr, "FP64 golden failed: %s\n", nbody_hip_last_error());
             return 1;
@@ -181,7 +235,7 @@ int main(int argc, char** argv)
     int idxFirstAlgorithm = 0, bPrintListOnly = 0, bUseGraphics = 0, bVerbose = 0;
     unsigned nSeed = kDefaultSeed;
     const char* ic = "reference";
-    enum { OPT_IC = 1000 };
+    enum { OPT_IC = 1000, OPT_GOLDEN };
     static const struct option cli_options[] = {
         {"bodies", required_argument, nullptr, 'n'},     {"gpus", required_argument, nullptr, 'g'},
         {"no-cpu", no_argument, &g_bNoCPU, 1},           {"no-crosscheck", no_argument, &g_bCrossCheck, 0},
@@ -189,7 +243,8 @@ int main(int argc, char** argv)
         {"seed", required_argument, nullptr, 's'},       {"list", no_argument, nullptr, 'l'},
         {"algorithm", required_argument, nullptr, 'a'},  {"graphics", no_argument, nullptr, 'G'},
         {"verbose", no_argument, nullptr, 'v'},          {"help", no_argument, nullptr, 'h'},
-        {"ic", required_argument, nullptr, OPT_IC},      {nullptr, 0, nullptr, 0}};
+        {"ic", required_argument, nullptr, OPT_IC},      {"golden", required_argument, nullptr, OPT_GOLDEN},
+        {nullptr, 0, nullptr, 0}};
 
     if (hipGetDeviceCount(&g_maxGPUs) != hipSuccess) g_maxGPUs = 0;
     g_numGPUs = g_maxGPUs;
@@ -304,6 +359,13 @@ int main(int argc, char** argv)
             }
             ic = optarg;
             break;
+        case OPT_GOLDEN:
+            if (strcmp(optarg, "fp64") && strcmp(optarg, "cpu_soa")) {
+                fprintf(stderr, "ERROR: --golden must be fp64 or cpu_soa\n");
+                return 1;
+            }
+            g_bGoldenSOA = !strcmp(optarg, "cpu_soa");
+            break;
         case 'h':
         case '?': print_usage(); return 1;
         }
@@ -327,6 +389,7 @@ int main(int argc, char** argv)
     const size_t n = s.n;
     fprintf(stderr, "Running simulation with %u particles", (unsigned)n);
     if (g_bCrossCheck) fprintf(stderr, ", crosscheck enabled");
+    if (g_bCrossCheck && g_bGoldenSOA) fprintf(stderr, " (CPU_SOA golden)");
     if (g_bNoCPU) fprintf(stderr, ", CPU disabled");
     if (g_numGPUs) fprintf(stderr, ", up to %u GPUs", g_numGPUs);
     fprintf(stderr, "\n");

@@ -36,6 +36,7 @@ def test_cli_exists():
     (["--algorithm", "GPU_Shared"], "Invalid algorithm name 'GPU_Shared'"),
     (["--algorithm", "99"], "Invalid algorithm index '99'"),
     (["--ic", "cube"], "--ic must be reference, ball or plummer"),
+    (["--golden", "cpu"], "--golden must be fp64 or cpu_soa"),
 ])
 def test_cli_rejects_bad_arguments(args, msg):
     r = run(*args)
@@ -44,7 +45,7 @@ def test_cli_rejects_bad_arguments(args, msg):
 
 def test_cli_help_and_list():
     r = run("--help")
-    assert r.returncode == 1 and "--bodies=<N>" in r.stderr and "--algorithm" in r.stderr
+    assert r.returncode == 1 and "--bodies=<N>" in r.stderr and "--algorithm" in r.stderr and "--golden" in r.stderr
     r = run("--list")
     assert r.returncode == 0
     listed = re.findall(r"^\s+(\d+) - (\w+)", r.stdout, re.M)
@@ -84,3 +85,24 @@ def test_cli_ball_and_plummer_run():
         r = run("--bodies", "2", "--iterations", "3", "--algorithm", "HIP", "--ic", ic, "--verbose", timeout=300)
         assert r.returncode == 0, r.stderr
         assert len([l for l in r.stdout.replace("\r", "\n").splitlines() if LINE.match(l)]) == 3
+
+
+@pytest.mark.gpu
+def test_cli_cpu_soa_golden():
+    # --golden=cpu_soa: the reference's own crosscheck golden (src/nbody.cu:311-321,
+    # CPU_SOA), so "Rel. error" is the figure the reference's logs print.
+    errs = {}
+    for golden in ("fp64", "cpu_soa"):
+        r = run("--bodies", "2", "--iterations", "2", "--algorithm", "HIP", "--golden", golden, "--verbose",
+                timeout=300)
+        assert r.returncode == 0, r.stderr
+        if golden == "cpu_soa":
+            assert "crosscheck enabled (CPU_SOA golden)" in r.stderr
+        ms = [LINE.match(l) for l in r.stdout.replace("\r", "\n").splitlines() if LINE.match(l)]
+        assert len(ms) == 2 and all(m.group("err") is not None for m in ms)
+        errs[golden] = [abs(float(m.group("err"))) for m in ms]
+    # both goldens agree with the GPU force to the FP32 rounding of the sum
+    # (componentwise, so small components inflate it; the FP64 figure bounds
+    # how far the FP32 golden itself can sit from the GPU's result)
+    for e64, esoa in zip(errs["fp64"], errs["cpu_soa"]):
+        assert esoa < 5e-2 and e64 < 5e-2, errs
