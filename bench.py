@@ -26,9 +26,12 @@ in one run, src/nbody.cu:1024-1053):
                 CPU_AOS with its stated tolerance and a pass flag;
   general_mass  the reference's own ICs (randomUnitBodies, m in U(0.5, 1),
                 src/nbody_util.cpp:137), which take the general pair weights;
-  ensemble      512 independent systems, batched graph vs one launch per system;
+  ensemble      512 independent systems, batched graph vs one launch per system,
+                and the batched force on the initial and final states of every
+                system against the FP64 direct sum;
   served        a Poisson stream of simulation requests: p50/p99 latency and
-                interactions/s;
+                interactions/s, and the force error on the final states of
+                sampled requests (replayed untimed after the timed run);
   plummer       N = 2^20 Plummer sphere: default vs plain vs compensated sums;
   cpu_baseline  the reference's CPU_SOA_tiled (src/nbody_CPU_SOA_tiled.cpp:51-115)
                 compiled from its sources, else the oracle's restatement, on
@@ -65,7 +68,9 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--ipl", type=int, default=0, help="pin i bodies per lane (skips the autotuner)")
     ap.add_argument("--jslice", type=int, default=0, help="pin the j slice length (skips the autotuner)")
-    ap.add_argument("--compensated", type=int, default=-1, choices=[-1, 0, 1, 2])
+    ap.add_argument("--compensated", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="accumulation of the single system's plan (2: FP64); with --workload served, the "
+                         "server's (2: FP64, else FP32 block partials)")
     ap.add_argument("--waves", type=int, default=0, help="pin the symmetric plan's waves per SIMD (skips the autotuner)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration for --pmc runs)")
@@ -311,13 +316,59 @@ def ensemble_systems(count):
     return sizes, [ics.uniform_ball(int(n), seed=100000 + k) for k, n in enumerate(sizes)]
 
 
+def batched_force_errors(sizes, pm, vm, eps2, compensated=-1):
+    """Per-body |F - F64|/|F64| of the batched force path (FP64 accumulation
+    with compensated = 2) on the given states: the systems (bodies back to
+    back, `sizes[k]` each) form one ensemble, one batched step computes each
+    system's force on these positions, and
+    the FP64 direct sum (GPU diagnostic) of every system is the reference.
+    Returns (max, rms) over all bodies of all systems."""
+    import numpy as np
+    import torch
+    from nbody_amd import _native
+    L = _native.lib()
+    nsz = np.ascontiguousarray(np.asarray(sizes, np.uint64))
+    e2 = np.full(len(nsz), eps2, np.float32)
+    pm = np.ascontiguousarray(pm, np.float32)
+    vm = np.ascontiguousarray(vm, np.float32)
+    total = int(nsz.sum())
+    opt = _native.Options.make(compensated=compensated)
+    h = L.nbody_hip_ensemble_create_opt(len(nsz), nsz.ctypes.data, pm.ctypes.data, vm.ctypes.data,
+                                        e2.ctypes.data, 0.016, 0.95, ctypes.byref(opt))
+    if not h:
+        raise _native.NativeError(_native.last_error())
+    force = np.zeros((total, 4), np.float32)
+    try:
+        _native.check(L.nbody_hip_ensemble_run(h, 1, _native.BATCHED, None), "ensemble step")
+        _native.check(L.nbody_hip_ensemble_read(h, None, None, force.ctypes.data), "ensemble read")
+    finally:
+        L.nbody_hip_ensemble_destroy(h)
+    dev = torch.device("cuda:0")
+    pm_d = torch.from_numpy(pm).to(dev)
+    f64 = torch.zeros((total, 4), dtype=torch.float64, device=dev)
+    off = 0
+    for n in nsz.tolist():
+        _native.check_ms(L.nbody_hip_force_fp64(ctypes.c_void_p(f64.data_ptr() + 32 * off),
+                                                ctypes.c_void_p(pm_d.data_ptr() + 16 * off), eps2, int(n)),
+                         "fp64")
+        off += int(n)
+    torch.cuda.synchronize()
+    ref = f64[:, :3].cpu().numpy()
+    d = np.linalg.norm(force[:, :3].astype(np.float64) - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    del f64, pm_d
+    return float(d.max()), float(np.sqrt(np.mean(d * d)))
+
+
 def run_ensemble(count, steps, warmup):
     """`count` independent uniform balls, N log-uniform in [256, 16384]
     (seed 4), eps^2 = 0.01, dt = 0.016: batched steps replayed from a graph
     (the systems in two groups, each a persistent force launch and a
     slot-sum-and-integrate launch on its own stream), the same launches
     without a graph, and every system run on its own with its best
-    single-system plan."""
+    single-system plan; then the batched graph and the per-system runs again
+    with FP64 accumulation (compensated = 2). Each batched graph's force on
+    its final state (and the FP32 one's on the initial state) is compared
+    with the FP64 direct sum over every body."""
     import numpy as np
     from nbody_amd import _native
     L = _native.lib()
@@ -327,11 +378,14 @@ def run_ensemble(count, steps, warmup):
     nsz = np.ascontiguousarray(sizes.astype(np.uint64))
     e2 = np.full(len(systems), 0.01, np.float32)
     inter = float((sizes.astype(np.float64) ** 2).sum())
-    res = {}
-    for name, mode in (("batched_graph", _native.BATCHED_GRAPH), ("batched", _native.BATCHED),
-                       ("per_system", _native.PER_SYSTEM)):
-        h = L.nbody_hip_ensemble_create(len(systems), nsz.ctypes.data, pm.ctypes.data, vm.ctypes.data,
-                                        e2.ctypes.data, 0.016, 0.95)
+    res, ends = {}, {}
+    for name, mode, comp in (("batched_graph", _native.BATCHED_GRAPH, -1), ("batched", _native.BATCHED, -1),
+                             ("per_system", _native.PER_SYSTEM, -1),
+                             ("batched_graph_fp64", _native.BATCHED_GRAPH, 2),
+                             ("per_system_fp64", _native.PER_SYSTEM, 2)):
+        opt = _native.Options.make(compensated=comp)
+        h = L.nbody_hip_ensemble_create_opt(len(systems), nsz.ctypes.data, pm.ctypes.data, vm.ctypes.data,
+                                            e2.ctypes.data, 0.016, 0.95, ctypes.byref(opt))
         if not h:
             raise _native.NativeError(_native.last_error())
         try:
@@ -340,9 +394,19 @@ def run_ensemble(count, steps, warmup):
             _native.check(L.nbody_hip_ensemble_run(h, steps, mode, ctypes.byref(ms)), name)
             res[name] = {"ms_per_step": round(ms.value / steps, 4),
                          "value": round(inter * steps / (ms.value * 1e-3) / 1e9, 2)}
+            if name.startswith("batched_graph"):
+                end_pm, end_vm = np.zeros_like(pm), np.zeros_like(vm)
+                _native.check(L.nbody_hip_ensemble_read(h, end_pm.ctypes.data, end_vm.ctypes.data, None), "read")
+                ends[name] = (end_pm, end_vm)
         finally:
             L.nbody_hip_ensemble_destroy(h)
+    ic_max, ic_rms = batched_force_errors(sizes, pm, vm, 0.01)
+    end_max, end_rms = batched_force_errors(sizes, *ends["batched_graph"], 0.01)
+    end64_max, end64_rms = batched_force_errors(sizes, *ends["batched_graph_fp64"], 0.01, compensated=2)
     v = res["batched_graph"]["value"]
+    v64 = res["batched_graph_fp64"]["value"]
+    state = f"the batched force on every system after the {warmup} warm-up and {steps} timed steps"
+    vs = "FP64 direct sum (GPU diagnostic) of every system, per-body |F-F64|/|F64| over all bodies"
     return {
         "value": v,
         "unit": "10^9 interactions/s",
@@ -353,6 +417,19 @@ def run_ensemble(count, steps, warmup):
         "modes": res,
         "speedup_vs_per_system": round(v / res["per_system"]["value"], 2),
         "achieved_tflops": round(20 * v / 1e3, 2),
+        "accuracy": {
+            "max_rel": end_max, "rms_rel": end_rms, "tolerance": 1e-5, "pass": end_max <= 1e-5,
+            "vs": vs, "state": state,
+            "initial_state": {"max_rel": ic_max, "rms_rel": ic_rms, "pass": ic_max <= 1e-5},
+        },
+        "fp64_accumulation": {
+            "value": v64,
+            "ms_per_step": res["batched_graph_fp64"]["ms_per_step"],
+            "speedup_vs_per_system": round(v64 / res["per_system_fp64"]["value"], 2),
+            "what": "compensated = 2 in both modes: every term added in FP64, FP64 block partials, one rounding",
+            "accuracy": {"max_rel": end64_max, "rms_rel": end64_rms, "tolerance": 1e-5, "pass": end64_max <= 1e-5,
+                         "vs": vs, "state": state},
+        },
     }
 
 
@@ -361,7 +438,7 @@ def served_trace(count, rate):
     return serve.poisson_trace(count, rate, 1024, 131072, 10, 200, seed=5)
 
 
-def run_served(count, rate, max_active):
+def run_served(count, rate, max_active, compensated=-1):
     """A seeded Poisson stream (mean `rate` requests/s) of simulations with N
     log-uniform in [1024, 131072], steps uniform in [10, 200], uniform-ball or
     Plummer ICs, eps^2 = 0.01, under a 200 GB HBM budget, replayed in real
@@ -370,7 +447,7 @@ def run_served(count, rate, max_active):
     latencies include the queue that builds up over the finite trace."""
     from nbody_amd import serve
     reqs = served_trace(count, rate)
-    srv = serve.Server(budget_bytes=200 * 10 ** 9, max_active=max_active)
+    srv = serve.Server(budget_bytes=200 * 10 ** 9, max_active=max_active, compensated=compensated)
     try:
         res, wall, work = srv.run(reqs)
         peak = srv.peak_active()
@@ -378,6 +455,7 @@ def run_served(count, rate, max_active):
         srv.close()
     summ = serve.latency_summary(res, wall, work)
     offered = sum(float(r.N) ** 2 * r.steps for r in reqs) / max(reqs[len(reqs) - 1].arrival_s, 1e-9)
+    acc = served_accuracy(reqs, compensated=compensated)
     return {
         "value": round(summ["interactions_per_s"] / 1e9, 2),
         "unit": "10^9 interactions/s",
@@ -390,11 +468,47 @@ def run_served(count, rate, max_active):
         "wall_s": round(wall, 3),
         "peak_active": peak,
         "workload": f"{count} requests, Poisson {rate:g}/s, N log-uniform [1024, 131072], steps U[10, 200], ball "
-                    f"or Plummer ICs, eps^2=0.01, dt=0.016, damping=0.95, 200 GB budget, {max_active} slots",
+                    f"or Plummer ICs, eps^2=0.01, dt=0.016, damping=0.95, 200 GB budget, {max_active} slots"
+                    + (", FP64 accumulation" if compensated == 2 else ""),
         "offered_interactions_per_s": offered,
         "offered_over_served": round(offered / max(summ["interactions_per_s"], 1.0), 3),
         "achieved_tflops": round(20 * summ["interactions_per_s"] / 1e12, 2),
+        "accuracy": acc,
     }, reqs
+
+
+def served_accuracy(reqs, every=25, compensated=-1):
+    """Force error on the final states of every `every`-th request of the
+    trace. A request's final state does not depend on what else is resident,
+    so these requests are replayed after the timed run, untimed, all
+    arriving at once into a server that keeps their states; the batched
+    force on each final state is then compared with the FP64 direct sum."""
+    import numpy as np
+    from nbody_amd import _native, serve
+    picks = list(range(0, len(reqs), every))
+    sub = (_native.Request * len(picks))()
+    for j, k in enumerate(picks):
+        r = reqs[k]
+        sub[j] = _native.Request(r.id, r.N, r.steps, r.seed, r.ic, r.softeningSquared, 0.0)
+    srv = serve.Server(budget_bytes=200 * 10 ** 9, max_active=len(picks), keep_states=True, compensated=compensated)
+    try:
+        res, _, _ = srv.run(sub)
+        if any(x.status != 0 for x in res):
+            raise RuntimeError("served accuracy replay: a request failed")
+        states = [srv.final_state(int(sub[j].id), int(sub[j].N)) for j in range(len(picks))]
+    finally:
+        srv.close()
+    eps2 = {float(sub[j].softeningSquared) for j in range(len(picks))}
+    if len(eps2) != 1:
+        raise RuntimeError("served accuracy replay: one softening expected")
+    sizes = [int(sub[j].N) for j in range(len(picks))]
+    mx, rms = batched_force_errors(sizes, np.concatenate([st[0] for st in states]),
+                                   np.concatenate([st[1] for st in states]), eps2.pop(), compensated=compensated)
+    return {"max_rel": mx, "rms_rel": rms, "tolerance": 1e-5, "pass": mx <= 1e-5,
+            "vs": "FP64 direct sum (GPU diagnostic), per-body |F-F64|/|F64| over all bodies of the sampled requests",
+            "sample": f"requests 0, {every}, {2 * every}, ... ({len(picks)} of {len(reqs)}; N {min(sizes)}.."
+                      f"{max(sizes)}, {sum(sizes)} bodies): the batched force on each final state, the requests "
+                      f"replayed untimed after the timed run"}
 
 
 def run_plummer(steps):
@@ -835,7 +949,7 @@ def main():
         print(json.dumps(standalone(body, "synthetic (seeded uniform balls, unit masses)", "ensemble")))
         return
     if args.workload == "served":
-        body, _ = run_served(args.requests, args.rate, args.max_active)
+        body, _ = run_served(args.requests, args.rate, args.max_active, args.compensated)
         print(json.dumps(standalone(body, "synthetic (seeded uniform-ball / Plummer requests)", "served")))
         return
     if args.workload == "plummer":

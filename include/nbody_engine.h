@@ -59,14 +59,26 @@ enum {
  * (system s starts at body sum_{r<s} N[r]); softeningSquared[s] per system;
  * dt and damping shared. The batched modes evaluate a system of N <= 2^18
  * bodies pairwise-symmetrically in blocks of 512, each pair once (block
- * pairs exactly as ComputeGravitation_HIP_Symmetric with 8 bodies per lane;
- * each diagonal block as one task, where that entry sums it one-sidedly, so
- * the two agree to rounding), keeping 12 * ceil(N/512)^2 * 512 bytes of
+ * pairs exactly as nbody_hip_force's symmetric plan with 8 bodies per lane
+ * and blocks of 512; each diagonal block as one task, where that plan sums
+ * it one-sidedly, so the two agree to rounding), keeping 12 * ceil(N/512)^2 * 512 bytes of
  * block partials in HBM; a larger one one-sidedly with compensated sums
  * (12 N bytes of partials). */
 nbody_hip_ensemble *nbody_hip_ensemble_create(size_t count, const size_t *N, const float *posMass,
                                               const float *velInvMass, const float *softeningSquared, float dt,
                                               float damping);
+/* The same with options (NULL: as nbody_hip_ensemble_create). opt->compensated
+ * = 2 selects FP64 accumulation for every mode: the batched modes then run
+ * the FP64 symmetric plan's pieces (4 bodies per lane, blocks of 256, FP64
+ * block partials, 24 * ceil(N/256)^2 * 256 bytes per system of N <= 2^17;
+ * a larger system one-sidedly with FP64 sums, 24 N bytes), so a system of
+ * 1024 .. 2^17 bodies gets bit for bit the forces of nbody_hip_force with
+ * compensated = 2; the per-system mode plans each system with opt. Other
+ * option fields apply to the per-system mode only. No counterpart in the
+ * reference, whose systems run one at a time (src/nbody.cu:1015-1088). */
+nbody_hip_ensemble *nbody_hip_ensemble_create_opt(size_t count, const size_t *N, const float *posMass,
+                                                  const float *velInvMass, const float *softeningSquared, float dt,
+                                                  float damping, const nbody_hip_options *opt);
 /* Runs `steps` steps of every system in `mode` and waits; *ms as above. In
  * the batched modes each body's force comes from the same arithmetic
  * whatever the other systems are, so a system gives bit-identical results

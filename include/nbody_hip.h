@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define NBODY_HIP_ABI_VERSION 3
+#define NBODY_HIP_ABI_VERSION 4
 
 /* Variant ids for nbody_hip_options.variant. */
 enum {

@@ -58,10 +58,17 @@ typedef struct nbody_hip_server nbody_hip_server;
  * each finished request's final posMass/velInvMass on the host. */
 nbody_hip_server *nbody_hip_server_create(size_t budget_bytes, int max_active, float dt, float damping,
                                           int keep_states);
+/* The same with options (NULL: as nbody_hip_server_create): opt->compensated
+ * = 2 steps every request with FP64 accumulation, the batched FP64 mode of
+ * nbody_hip_ensemble_create_opt; no other field is read. */
+nbody_hip_server *nbody_hip_server_create_opt(size_t budget_bytes, int max_active, float dt, float damping,
+                                              int keep_states, const nbody_hip_options *opt);
 
 /* Device bytes a resident request of N bodies holds, the amount the budget
  * counts: 48 N (posMass, velInvMass, force) plus its block partials. */
 size_t nbody_hip_request_bytes(size_t N);
+/* The same for a server created with opt (FP64 partials with compensated = 2). */
+size_t nbody_hip_request_bytes_opt(size_t N, const nbody_hip_options *opt);
 
 /* Replays a trace: request k enters the queue when arrival_s seconds have
  * passed since the call started; returns when every request has finished or

@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libnbody_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # bytes of the fixed device array of nbody_hip_force_noalloc (NBODY_HIP_NOALLOC_SCRATCH_BYTES)
 NOALLOC_SCRATCH_BYTES = 64 << 20
@@ -102,15 +102,18 @@ SIGNATURES = {
     "nbody_hip_system_plan": (_i, [_p, _OPT]),
     "nbody_hip_system_destroy": (None, [_p]),
     "nbody_hip_ensemble_create": (_p, [_sz, _p, _p, _p, _p, _f, _f]),
+    "nbody_hip_ensemble_create_opt": (_p, [_sz, _p, _p, _p, _p, _f, _f, _OPT]),
     "nbody_hip_ensemble_run": (_i, [_p, _i, _i, ctypes.POINTER(_f)]),
     "nbody_hip_ensemble_read": (_i, [_p, _p, _p, _p]),
     "nbody_hip_ensemble_destroy": (None, [_p]),
     "nbody_hip_server_create": (_p, [_sz, _i, _f, _f, _i]),
+    "nbody_hip_server_create_opt": (_p, [_sz, _i, _f, _f, _i, _OPT]),
     "nbody_hip_server_run_trace": (_i, [_p, ctypes.POINTER(Request), _sz, ctypes.POINTER(Result),
                                         ctypes.POINTER(_d), ctypes.POINTER(_d)]),
     "nbody_hip_server_final_state": (_i, [_p, ctypes.c_uint64, _p, _p]),
     "nbody_hip_server_peak_active": (_i, [_p]),
     "nbody_hip_request_bytes": (_sz, [_sz]),
+    "nbody_hip_request_bytes_opt": (_sz, [_sz, _OPT]),
     "nbody_hip_server_destroy": (None, [_p]),
     "nbody_random_unit_bodies": (_i, [_p, _p, _sz, _u32, _f, _f]),
     "nbody_uniform_ball": (_i, [_p, _p, _sz, _u32, _f, _f, _f]),

@@ -67,6 +67,11 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
 // 12 N * blocks bytes; past kSymBandCap they are kept one column band at a
 // time).
 constexpr int kSymSlotsTarget = 512;
+// Below N = 512 * kSmallSymBlocks the default symmetric plan takes smaller
+// blocks (256, then 128 bodies below N = 256 * kSmallSymBlocks), keeping
+// about this many: at N = 8192, 16384 and 32768 the autotuner's choices
+// (blocks of 128, 256, 512) all have 64.
+constexpr int kSmallSymBlocks = 64;
 
 // Default decomposition: enough blocks for ~48 waves per SIMD (the dispatcher
 // balances them across the 256 CUs), j slices of at least 1024 bodies in
@@ -89,10 +94,17 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
     }
     if (variant == VAR_SYMMETRIC) {
         // blocks of B = 64 * ipl * waves (default 8 bodies per lane and the
-        // fewest waves that keep at most kSymSlotsTarget blocks: B = 512 up to
-        // N = 256K, 2048 at N = 1M); the block slots are always summed with TwoSum
+        // fewest waves that keep at most kSymSlotsTarget blocks: B = 512 from
+        // N = 32K up to 256K, 2048 at N = 1M; smaller systems below); the block
+        // slots are always summed with TwoSum
         p.ipl = (opt && opt->i_per_lane > 0) ? opt->i_per_lane : 8;
-        if (opt && opt->jslice > 0) {
+        const bool shape_given = opt && (opt->i_per_lane > 0 || opt->jslice > 0);
+        if (!shape_given && N < (size_t)kSmallSymBlocks * 512) {
+            // small systems: keep about kSmallSymBlocks blocks (the shapes the
+            // autotuner picks there), so the pair tasks fill the chip
+            p.ipl = N < (size_t)kSmallSymBlocks * 256 ? 2 : 4;
+            p.jslice = 64 * p.ipl;
+        } else if (opt && opt->jslice > 0) {
             p.jslice = opt->jslice;
         } else if (p.ipl == 2) {
             p.jslice = 512;

@@ -115,19 +115,15 @@ __device__ __forceinline__ void sym_pair64(const float4* __restrict__ pm, int n,
     }
 }
 
-// Piece k (of A) of block P's diagonal: 64 bodies P*B + 64k + lane against
-// the bodies of block P, one-sided, in j order, into slot P. Lanes past n
-// compute on a zero body and store into padding that no reduction reads.
-template <int B, bool NEWTON>
-__device__ __forceinline__ void sym_diag_piece64(const float4* __restrict__ pm, int n, float eps2,
-                                                 const SlotMap64& slots, int P, int k)
+// Body i (this lane's) against bodies [j0, j0 + len), one-sided, in j order,
+// every term added in FP64 into (ax, ay, az). A lane whose i is past n
+// computes on a zero body.
+template <bool NEWTON>
+__device__ __forceinline__ void row64(const float4* __restrict__ pm, int n, float eps2, int i, int j0, int len,
+                                      double& ax, double& ay, double& az)
 {
-    if (P * B + k * 64 >= n) return;  // a piece made only of padding (wave-uniform)
-    const int i = P * B + k * 64 + (int)threadIdx.x;
     const float4 q = i < n ? pm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int j0 = P * B;
-    const int len = min(n - j0, B);
-    double ax = 0.0, ay = 0.0, az = 0.0;
+    ax = ay = az = 0.0;
     for (int j = 0; j < len; ++j) {
         const float4 b = pm[j0 + j];
         const float dx = b.x - q.x;
@@ -142,6 +138,20 @@ __device__ __forceinline__ void sym_diag_piece64(const float4* __restrict__ pm, 
         ay = __builtin_fma((double)dy, w, ay);
         az = __builtin_fma((double)dz, w, az);
     }
+}
+
+// Piece k (of A) of block P's diagonal: 64 bodies P*B + 64k + lane against
+// the bodies of block P, one-sided, in j order, into slot P. Lanes past n
+// compute on a zero body and store into padding that no reduction reads.
+template <int B, bool NEWTON>
+__device__ __forceinline__ void sym_diag_piece64(const float4* __restrict__ pm, int n, float eps2,
+                                                 const SlotMap64& slots, int P, int k)
+{
+    if (P * B + k * 64 >= n) return;  // a piece made only of padding (wave-uniform)
+    const int i = P * B + k * 64 + (int)threadIdx.x;
+    const int j0 = P * B;
+    double ax, ay, az;
+    row64<NEWTON>(pm, n, eps2, i, j0, min(n - j0, B), ax, ay, az);
     double* __restrict__ dst = slots.j_side(P, P, B) + k * 64 + threadIdx.x;
     dst[0] = ax;
     dst[slots.bplane] = ay;

@@ -127,7 +127,8 @@ struct nbody_hip_ensemble {
     float4* pm = nullptr;
     float4* vm = nullptr;
     float4* force = nullptr;
-    float* part = nullptr;         // every system's slot planes, back to back
+    char* part = nullptr;          // every system's slot planes, back to back
+    bool acc64 = false;            // FP64 accumulation (compensated = 2)
     // The systems in kGroups groups of about equal work, each with its own
     // task lists and work-queue counter; a batched step forks, runs each
     // group's force and slot-sum launches on its own stream, and joins, so
@@ -169,8 +170,8 @@ struct nbody_hip_ensemble {
     int queue_group(int g, hipStream_t s)
     {
         unsigned* c = counter + g * kCounterStride;
-        HIP_TRY(launch_force_batch(ftasks[g], nft[g], c, blocks, s));
-        HIP_TRY(launch_reduce_integrate_batch(btasks[g], nbt[g], dt, damping, c, s));
+        HIP_TRY(launch_force_batch(ftasks[g], nft[g], c, blocks, s, acc64));
+        HIP_TRY(launch_reduce_integrate_batch(btasks[g], nbt[g], dt, damping, c, s, acc64));
         return OK;
     }
 
@@ -319,6 +320,13 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
                                               const float* velInvMass, const float* softeningSquared, float dt,
                                               float damping)
 {
+    return nbody_hip_ensemble_create_opt(count, N, posMass, velInvMass, softeningSquared, dt, damping, nullptr);
+}
+
+nbody_hip_ensemble* nbody_hip_ensemble_create_opt(size_t count, const size_t* N, const float* posMass,
+                                                  const float* velInvMass, const float* softeningSquared, float dt,
+                                                  float damping, const nbody_hip_options* opt)
+{
     std::lock_guard<std::mutex> lock(g_mutex);
     if (count == 0 || !N || !posMass || !velInvMass || !softeningSquared) {
         set_error("empty ensemble or null argument");
@@ -330,6 +338,7 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
     e->count = count;
     e->dt = dt;
     e->damping = damping;
+    e->acc64 = opt && opt->compensated == 2;
     e->n.assign(N, N + count);
     e->eps2.assign(softeningSquared, softeningSquared + count);
     e->off.resize(count);
@@ -349,17 +358,17 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
         delete e;
         return nullptr;
     }
-    std::vector<size_t> part_off(count);
+    std::vector<size_t> part_off(count);  // bytes
     size_t part_total = 0;
     for (size_t s = 0; s < count; ++s) {
         part_off[s] = part_total;
-        part_total += batch_part_floats(N[s]);
+        part_total += batch_part_bytes(N[s], e->acc64);
     }
-    e->blocks = d->cus * batch_force_blocks_per_cu();
+    e->blocks = d->cus * batch_force_blocks_per_cu(e->acc64);
     size_t scratch = 0;
     e->plans.resize(count);
     for (size_t s = 0; s < count; ++s) {
-        if (make_plan(*d, N[s], N[s], nullptr, &e->plans[s])) {
+        if (make_plan(*d, N[s], N[s], opt, &e->plans[s])) {
             delete e;
             return nullptr;
         }
@@ -401,9 +410,10 @@ nbody_hip_ensemble* nbody_hip_ensemble_create(size_t count, const size_t* N, con
             std::vector<BodyTask> bt;
             for (size_t s = 0; s < count; ++s) {
                 if (group[s] != g) continue;
-                append_force_tasks(ft, e->pm + e->off[s], e->part + part_off[s], N[s], softeningSquared[s]);
+                append_force_tasks(ft, e->pm + e->off[s], e->part + part_off[s], N[s], softeningSquared[s],
+                                   e->acc64);
                 append_body_tasks(bt, e->pm + e->off[s], e->vm + e->off[s], e->force + e->off[s],
-                                  e->part + part_off[s], N[s]);
+                                  e->part + part_off[s], N[s], e->acc64);
             }
             order_force_tasks(ft);
             e->nft[g] = (int)ft.size();
@@ -503,11 +513,12 @@ struct ServeReq {
     bool reserved = false;  // its bytes count against the budget
     int ics = 0;            // host initial conditions: 0 not built, 1 built, 2 failed
     int group = 0;          // which of the server's two launch groups steps it
+    bool acc64 = false;     // FP64 accumulation (the server's mode)
     std::vector<float> hpm, hvm;  // initial conditions built on the host, until uploaded
     ForceTaskBands tasks;         // its force tasks, built once it is on the device
     std::vector<BodyTask> body_tasks;
-    size_t bytes() const { return 3 * r.N * sizeof(float4) + batch_part_floats(r.N) * sizeof(float); }
-    float* part() const { return reinterpret_cast<float*>(buf + 3 * r.N); }
+    size_t bytes() const { return 3 * r.N * sizeof(float4) + batch_part_bytes(r.N, acc64); }
+    void* part() const { return buf + 3 * r.N; }
 };
 
 // Seeded initial conditions of a request (nbody_ics.h).
@@ -527,6 +538,7 @@ struct nbody_hip_server {
     int max_active = 1;
     float dt = 0.f, damping = 1.f;
     bool keep = false;
+    bool acc64 = false;            // FP64 accumulation (compensated = 2)
     hipStream_t stream = nullptr;  // the steps (group 0)
     hipStream_t side = nullptr;    // group 1's launches
     hipStream_t up = nullptr;      // uploads during a step (non-blocking: it does not wait for the step)
@@ -565,6 +577,12 @@ extern "C" {
 nbody_hip_server* nbody_hip_server_create(size_t budget_bytes, int max_active, float dt, float damping,
                                           int keep_states)
 {
+    return nbody_hip_server_create_opt(budget_bytes, max_active, dt, damping, keep_states, nullptr);
+}
+
+nbody_hip_server* nbody_hip_server_create_opt(size_t budget_bytes, int max_active, float dt, float damping,
+                                              int keep_states, const nbody_hip_options* opt)
+{
     std::lock_guard<std::mutex> lock(g_mutex);
     if (budget_bytes == 0 || max_active < 1) {
         set_error("server needs a nonzero budget and at least one slot");
@@ -578,7 +596,8 @@ nbody_hip_server* nbody_hip_server_create(size_t budget_bytes, int max_active, f
     s->dt = dt;
     s->damping = damping;
     s->keep = keep_states != 0;
-    s->blocks = d->cus * batch_force_blocks_per_cu();
+    s->acc64 = opt && opt->compensated == 2;
+    s->blocks = d->cus * batch_force_blocks_per_cu(s->acc64);
     hipError_t e = hipStreamCreate(&s->stream);
     if (e == hipSuccess) e = hipStreamCreate(&s->side);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
@@ -612,6 +631,7 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     std::vector<size_t> order(count);
     for (size_t k = 0; k < count; ++k) {
         rq[k].r = reqs[k];
+        rq[k].acc64 = s->acc64;
         order[k] = k;
     }
     std::stable_sort(order.begin(), order.end(),
@@ -703,11 +723,12 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
                 set_error("admitting request %llu failed: %s", (unsigned long long)q->r.id, hipGetErrorString(e));
                 return E_HIP;
             }
-            build_force_task_bands(q->tasks, q->buf, q->part(), q->r.N, q->r.softeningSquared);
+            build_force_task_bands(q->tasks, q->buf, q->part(), q->r.N, q->r.softeningSquared, q->acc64);
             q->group = load[1] < load[0] ? 1 : 0;
             load[q->group] += (double)q->r.N * (double)q->r.N;
             q->body_tasks.clear();
-            append_body_tasks(q->body_tasks, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->part(), q->r.N);
+            append_body_tasks(q->body_tasks, q->buf, q->buf + q->r.N, q->buf + 2 * q->r.N, q->part(), q->r.N,
+                              q->acc64);
         }
         staged.clear();
         return OK;
@@ -736,9 +757,9 @@ int nbody_hip_server_run_trace(nbody_hip_server* s, const nbody_hip_request* req
     auto launch_step = [&](int k) -> hipError_t {
         auto group = [&](int g, hipStream_t st) {
             unsigned* c = s->counter + g * nbody_hip_server::kCounterStride;
-            hipError_t e = launch_force_batch(s->ftasks[k][g], nft[k][g], c, s->blocks, st);
+            hipError_t e = launch_force_batch(s->ftasks[k][g], nft[k][g], c, s->blocks, st, s->acc64);
             if (e == hipSuccess)
-                e = launch_reduce_integrate_batch(s->btasks[k][g], nbt[k][g], s->dt, s->damping, c, st);
+                e = launch_reduce_integrate_batch(s->btasks[k][g], nbt[k][g], s->dt, s->damping, c, st, s->acc64);
             return e;
         };
         if (nft[k][0] == 0 || nft[k][1] == 0) {
@@ -890,10 +911,13 @@ int nbody_hip_server_final_state(const nbody_hip_server* s, uint64_t id, float* 
 
 int nbody_hip_server_peak_active(const nbody_hip_server* s) { return s ? s->peak_active : 0; }
 
-size_t nbody_hip_request_bytes(size_t N)
+size_t nbody_hip_request_bytes(size_t N) { return nbody_hip_request_bytes_opt(N, nullptr); }
+
+size_t nbody_hip_request_bytes_opt(size_t N, const nbody_hip_options* opt)
 {
     ServeReq q;
     q.r.N = N;
+    q.acc64 = opt && opt->compensated == 2;
     return q.bytes();
 }
 

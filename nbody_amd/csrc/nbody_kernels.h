@@ -113,24 +113,43 @@ namespace nbody {
 // single slot. A second launch adds each body's slots in slot order (TwoSum)
 // and integrates it. A body's result therefore depends only on its own
 // system, never on what else is in the batch or on scheduling.
+//
+// A batch with FP64 accumulation (compensated = 2) runs the FP64 symmetric
+// plan's pieces instead (nbody_acc64.h): blocks of kBatchBlock64 (4 bodies
+// per lane, one wave), PAIR64 tasks, four DIAG64 pieces per diagonal block
+// (64 bodies one-sided against the block), FP64 slot planes summed in slot
+// order and rounded once; a system above kBatchPairMaxN64 runs as ROW64
+// tasks (64 bodies against all n, one-sided, FP64 sums) into one slot. A
+// system of 1024 .. kBatchPairMaxN64 bodies then gets exactly the forces of
+// the single-system FP64 symmetric plan (4 bodies per lane, blocks of 256).
 
 constexpr int kBatchBlock = 512;                     // 8 bodies per lane, one wave
 constexpr size_t kBatchPairMaxN = size_t(1) << 18;   // largest system run as block pairs
+constexpr int kBatchBlock64 = 256;                   // FP64 accumulation: 4 bodies per lane, one wave
+constexpr size_t kBatchPairMaxN64 = size_t(1) << 17; // FP64 accumulation: largest system run as block pairs
 
-enum BatchTaskKind : int { TASK_PAIR = 0, TASK_DIAG = 1, TASK_ROW_COMP = 2 };
+enum BatchTaskKind : int {
+    TASK_PAIR = 0,
+    TASK_DIAG = 1,
+    TASK_ROW_COMP = 2,
+    TASK_PAIR64 = 3,
+    TASK_DIAG64 = 4,
+    TASK_ROW64 = 5,
+};
 
 struct ForceTask {
     const float4* pm;  // the system's bodies
-    float* part;       // the system's slot planes [3][nb][npad]
+    float* part;       // the system's slot planes [3][nb][npad] (FP32 tasks)
+    double* part64;    // the same in FP64 (FP64 tasks)
     int n;             // bodies in the system
     int nb;            // slots per body
     int npad;          // length of one slot row
     float eps2;
     int kind;          // BatchTaskKind
-    int a;             // PAIR, DIAG: block P; ROW_COMP: first i body (64 of them)
-    int b;             // PAIR: block Q; ROW_COMP: first j body
-    int len;           // DIAG: bodies in the block; ROW_COMP: number of j bodies
-    int slot;          // slot written (DIAG: P; ROW_COMP: 0)
+    int a;             // PAIR, DIAG: block P; ROW_COMP, ROW64: first i body (64 of them)
+    int b;             // PAIR: block Q; DIAG64: piece (0..3); ROW_COMP, ROW64: first j body
+    int len;           // DIAG, DIAG64: bodies in the block; ROW_COMP, ROW64: number of j bodies
+    int slot;          // slot written (DIAG, DIAG64: P; ROW_COMP, ROW64: 0)
     int reserved;
 };
 
@@ -139,7 +158,8 @@ struct BodyTask {
     float4* pm;
     float4* vm;
     float4* force;
-    const float* part;
+    const float* part;     // FP32 slot planes, or
+    const double* part64;  // FP64 slot planes
     int i0;
     int ni;
     int nb;
@@ -147,13 +167,15 @@ struct BodyTask {
 };
 
 // Slots per body and slot row length of a batched system of n bodies.
-void batch_layout(size_t n, int* nb, int* npad);
-// Floats of slot planes a batched system of n bodies needs (3 * nb * npad).
-size_t batch_part_floats(size_t n);
-// Append one system's force tasks / body tasks.
-void append_force_tasks(std::vector<ForceTask>& out, const float4* pm, float* part, size_t n, float eps2);
-void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float4* force, const float* part,
-                       size_t n);
+void batch_layout(size_t n, bool acc64, int* nb, int* npad);
+// Bytes of slot planes a batched system of n bodies needs (3 * nb * npad
+// floats, or doubles with FP64 accumulation).
+size_t batch_part_bytes(size_t n, bool acc64);
+// Append one system's force tasks / body tasks; `part` is the system's slot
+// planes (floats, or doubles when acc64).
+void append_force_tasks(std::vector<ForceTask>& out, const float4* pm, void* part, size_t n, float eps2, bool acc64);
+void append_body_tasks(std::vector<BodyTask>& out, float4* pm, float4* vm, float4* force, const void* part,
+                       size_t n, bool acc64);
 // Most expensive tasks first (stable), so the end of the launch holds the
 // cheapest ones: compensated rows by cost, then the block tasks by their
 // tiles (a block pair whose second block is a system's ragged last block
@@ -169,21 +191,23 @@ void order_force_tasks(std::vector<ForceTask>& tasks);
 struct ForceTaskBands {
     std::vector<ForceTask> big, pairs, diag;
 };
-void build_force_task_bands(ForceTaskBands& out, const float4* pm, float* part, size_t n, float eps2);
+void build_force_task_bands(ForceTaskBands& out, const float4* pm, void* part, size_t n, float eps2, bool acc64);
 void concat_force_task_bands(std::vector<ForceTask>& out, const std::vector<const ForceTaskBands*>& systems);
 // Resident one-wave force workgroups per CU (the persistent grid is this
-// times the CU count).
-int batch_force_blocks_per_cu();
+// times the CU count) of the FP32 or the FP64-accumulation force kernel.
+int batch_force_blocks_per_cu(bool acc64);
 
 // Persistent force launch: `blocks` workgroups each start with the task of
 // their own index, then take tasks from `counter` (which must read 0 at
-// launch) until all ntasks are done.
-hipError_t launch_force_batch(const ForceTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream);
+// launch) until all ntasks are done. The tasks are all FP32 kinds, or all
+// FP64 kinds with acc64.
+hipError_t launch_force_batch(const ForceTask* tasks, int ntasks, unsigned* counter, int blocks, hipStream_t stream,
+                              bool acc64);
 
 // Sum every body's slots, write force (x, y, z; w untouched), integrate
 // (dt, damping as integrateGravitation), and reset `counter` to 0 for the
 // next force launch.
 hipError_t launch_reduce_integrate_batch(const BodyTask* tasks, int ntasks, float dt, float damping,
-                                         unsigned* counter, hipStream_t stream);
+                                         unsigned* counter, hipStream_t stream, bool acc64);
 
 }  // namespace nbody

@@ -27,15 +27,23 @@ def poisson_trace(count, rate_hz, n_lo, n_hi, steps_lo, steps_hi, seed, eps2=0.0
     return reqs
 
 
-def request_bytes(n):
-    """Device bytes a resident request of n bodies holds (what the budget counts)."""
-    return int(_native.lib().nbody_hip_request_bytes(int(n)))
+def request_bytes(n, compensated=-1):
+    """Device bytes a resident request of n bodies holds (what the budget
+    counts); compensated=2 for a server with FP64 accumulation."""
+    opt = _native.Options.make(compensated=compensated)
+    return int(_native.lib().nbody_hip_request_bytes_opt(int(n), ctypes.byref(opt)))
 
 
 class Server:
-    def __init__(self, budget_bytes=200 * 10 ** 9, max_active=64, dt=0.016, damping=0.95, keep_states=False):
+    """compensated=2 steps every request with FP64 accumulation
+    (nbody_hip_server_create_opt); the default keeps FP32 block partials."""
+
+    def __init__(self, budget_bytes=200 * 10 ** 9, max_active=64, dt=0.016, damping=0.95, keep_states=False,
+                 compensated=-1):
         self.L = _native.lib()
-        self.h = self.L.nbody_hip_server_create(int(budget_bytes), max_active, dt, damping, 1 if keep_states else 0)
+        opt = _native.Options.make(compensated=compensated)
+        self.h = self.L.nbody_hip_server_create_opt(int(budget_bytes), max_active, dt, damping,
+                                                    1 if keep_states else 0, ctypes.byref(opt))
         if not self.h:
             raise _native.NativeError(_native.last_error())
 

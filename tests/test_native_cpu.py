@@ -155,6 +155,20 @@ def test_request_bytes_closed_form():
     assert serve.request_bytes(n) == 48 * n + 12 * n
 
 
+def test_request_bytes_fp64_closed_form():
+    from nbody_amd import serve
+    # FP64 accumulation: 3 double planes of ceil(N/256) slots x ceil(N/256)*256 bodies
+    for n in (1, 256, 257, 1000, 16384, 2 ** 17):
+        nb = -(-n // 256)
+        assert serve.request_bytes(n, compensated=2) == 48 * n + 24 * nb * nb * 256, n
+    # beyond 2^17 bodies one FP64 slot per body
+    n = 2 ** 17 + 1
+    assert serve.request_bytes(n, compensated=2) == 48 * n + 24 * n
+    # compensated 0 and 1 keep the FP32 partials
+    for c in (0, 1):
+        assert serve.request_bytes(5000, compensated=c) == serve.request_bytes(5000)
+
+
 def test_size_and_range_limits_fail_before_any_device_call():
     """Argument checks run before the library touches a device: N above the
     2^28 cap, N = 0, and i-ranges outside [0, N) return the failure value
