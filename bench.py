@@ -31,7 +31,8 @@ in one run, src/nbody.cu:1024-1053):
                 system against the FP64 direct sum;
   served        a Poisson stream of simulation requests: p50/p99 latency and
                 interactions/s, and the force error on the final states of
-                sampled requests (replayed untimed after the timed run);
+                sampled requests (replayed untimed after the timed run); the
+                same with FP64 accumulation;
   plummer       N = 2^20 Plummer sphere: default vs plain vs compensated sums;
   cpu_baseline  the reference's CPU_SOA_tiled (src/nbody_CPU_SOA_tiled.cpp:51-115)
                 compiled from its sources, else the oracle's restatement, on
@@ -967,6 +968,11 @@ def main():
         out["general_mass"] = run_general_mass(args.n, 20)
         out["ensemble"] = run_ensemble(args.systems, args.ensemble_steps, 5)
         out["served"], reqs = run_served(args.requests, args.rate, args.max_active)
+        served64, _ = run_served(args.requests, args.rate, args.max_active, compensated=2)
+        out["served"]["fp64_accumulation"] = {
+            k: served64[k] for k in ("value", "p50_latency_s", "p99_latency_s", "p99_queue_wait_s", "completed",
+                                     "wall_s", "offered_over_served", "accuracy")}
+        out["served"]["fp64_accumulation"]["what"] = "the same trace with compensated = 2: FP64 block partials"
         out["plummer"] = run_plummer(args.plummer_steps)
         ens_sizes, _ = ensemble_systems(args.systems)
     if not args.no_cpu_baseline:
