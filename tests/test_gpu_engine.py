@@ -110,7 +110,8 @@ def test_graph_replays_match_eager(opt):
 def test_step_crosscheck_every_step(oracle):
     """Ten steps of the N = 16384 uniform-ball workload, crosschecked at every
     step on identical positions against the oracle's CPU_AOS (the reference's
-    crosscheck target), and the integrator bit-exact against the oracle's."""
+    crosscheck target) and CPU_SOA (the reference driver's golden,
+    src/nbody.cu:313-325), and the integrator bit-exact against the oracle's."""
     n = 16384
     pm0, vm0 = ics.uniform_ball(n, seed=1)
     s = System(pm0, vm0, 0.01)
@@ -120,10 +121,14 @@ def test_step_crosscheck_every_step(oracle):
             s.run(1, graph=True)
             pm, vm, f = s.read()
             aos, _ = oracle.cpu_aos(prev_pm, np.float32(0.01))
+            soa, _ = oracle.cpu_soa(prev_pm, np.float32(0.01))
             f64 = oracle.fp64_forces(prev_pm, np.float32(0.01))
-            d = np.linalg.norm(f[:, :3].astype(np.float64) - aos[:, :3], axis=1) / np.linalg.norm(f64, axis=1)
+            norm = np.linalg.norm(f64, axis=1)
+            d = np.linalg.norm(f[:, :3].astype(np.float64) - aos[:, :3], axis=1) / norm
+            d_soa = np.linalg.norm(f[:, :3].astype(np.float64) - soa[:, :3], axis=1) / norm
             mx, rms = force_errors(f, f64)
-            assert d.max() <= 2e-5 and mx <= 1e-5 and rms <= 1e-6, (step, d.max(), mx, rms)
+            assert d.max() <= 2e-5 and d_soa.max() <= 2e-5 and mx <= 1e-5 and rms <= 1e-6, (
+                step, d.max(), d_soa.max(), mx, rms)
             p_ref, v_ref = oracle.integrate_aos(prev_pm, prev_vm, f, 0.016, 0.95)
             assert np.array_equal(bits(pm), bits(p_ref)) and np.array_equal(bits(vm), bits(v_ref)), step
             prev_pm, prev_vm = pm, vm
