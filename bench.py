@@ -122,21 +122,36 @@ def physical_cores():
 
 
 def cpu_model():
+    """lscpu's model name, with family / model / stepping and the widest x86
+    vector extension, since a virtual machine may report only a generic
+    name."""
     import subprocess
+    fields = {}
     try:
         out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, check=True).stdout
         for line in out.splitlines():
-            if line.startswith("Model name:"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
     except (OSError, subprocess.CalledProcessError):
         pass
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+    if "Model name" not in fields:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    k = {"model name": "Model name", "cpu family": "CPU family", "model": "Model",
+                         "stepping": "Stepping", "flags": "Flags"}.get(k.strip(), k.strip())
+                    fields.setdefault(k, v.strip())
+        except OSError:
+            pass
+    name = fields.get("Model name", "unknown")
+    flags = set(fields.get("Flags", "").split())
+    isa = "AVX-512" if "avx512f" in flags else "AVX2" if "avx2" in flags else None
+    extra = [f"{k.lower()} {fields[k]}" for k in ("CPU family", "Model", "Stepping") if k in fields]
+    if isa:
+        extra.append(isa)
+    return f"{name} ({', '.join(extra)})" if extra else name
 
 
 def _omp():

@@ -171,3 +171,21 @@ def test_c4_evolved_state_holds_the_bound_with_fp64_accumulation():
     print(f"512 systems after 55 steps: FP64 accumulation max {mx:.3e} rms {rms:.3e}; FP32 batched max "
           f"{rel32.max():.3e} rms {np.sqrt(np.mean(rel32 ** 2)):.3e}")
     assert mx <= TOL_MAX and rms <= TOL_RMS, (mx, rms)
+
+
+def test_options_the_fp64_batches_refuse():
+    # FP64 accumulation is built for the LDS and symmetric plans: an ensemble
+    # whose per-system plans would need another variant is refused at creation
+    L = _native.lib()
+    pm, vm = ics.uniform_ball(2000, seed=9)
+    n = np.array([2000], np.uint64)
+    e2 = np.array([EPS2], np.float32)
+    opt = _native.Options.make(variant=_native.SGPR, compensated=2)
+    h = L.nbody_hip_ensemble_create_opt(1, n.ctypes.data, pm.ctypes.data, vm.ctypes.data, e2.ctypes.data, DT,
+                                        DAMPING, ctypes.byref(opt))
+    assert not h and "FP64 accumulation" in _native.last_error()
+    # NULL options behave as nbody_hip_ensemble_create
+    h = L.nbody_hip_ensemble_create_opt(1, n.ctypes.data, pm.ctypes.data, vm.ctypes.data, e2.ctypes.data, DT,
+                                        DAMPING, None)
+    assert h, _native.last_error()
+    L.nbody_hip_ensemble_destroy(h)
