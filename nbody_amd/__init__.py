@@ -6,10 +6,11 @@ behind the C ABI of include/nbody_hip.h); this package is its Python binding:
 
     _native      ctypes declarations of every C entry point
     algorithms   the algorithm table (src/nbody.cu:83-103 shape)
+    engine       device-resident stepping: one system, or an ensemble
     ics          seeded initial conditions (reference-compatible, ball, Plummer)
     serve        request traces for the continuous-batching server
 """
-from . import _native, algorithms, ics, serve  # noqa: F401
+from . import _native, algorithms, engine, ics, serve  # noqa: F401
 from ._native import NativeError, NativeLibraryMissing, Options, lib  # noqa: F401
 
 SOFTENING = 0.1          # src/nbody.cu:151 (softeningSquared = 0.1^2)

@@ -19,6 +19,7 @@ from conftest import EPS2
 from oracle_lib import force_errors
 
 from nbody_amd import _native, ics
+from nbody_amd.engine import Ensemble, System
 
 pytestmark = pytest.mark.gpu
 L = None
@@ -33,60 +34,6 @@ def lib():
 
 def bits(a):
     return np.ascontiguousarray(a).view(np.uint32)
-
-
-class System:
-    def __init__(self, pm, vm, eps2, dt=0.016, damping=0.95, **opt):
-        self.n = pm.shape[0]
-        o = _native.Options.make(**opt) if opt else None
-        self.h = lib().nbody_hip_system_create(pm.ctypes.data, vm.ctypes.data, self.n, eps2, dt, damping,
-                                               ctypes.byref(o) if o else None)
-        assert self.h, _native.last_error()
-
-    def run(self, steps, graph):
-        ms = ctypes.c_float()
-        _native.check(lib().nbody_hip_system_run(self.h, steps, 1 if graph else 0, ctypes.byref(ms)), "run")
-        return ms.value
-
-    def read(self):
-        pm = np.zeros((self.n, 4), np.float32)
-        vm = np.zeros((self.n, 4), np.float32)
-        f = np.zeros((self.n, 4), np.float32)
-        _native.check(lib().nbody_hip_system_read(self.h, pm.ctypes.data, vm.ctypes.data, f.ctypes.data), "read")
-        return pm, vm, f
-
-    def close(self):
-        lib().nbody_hip_system_destroy(self.h)
-
-
-class Ensemble:
-    def __init__(self, systems, eps2, dt=0.016, damping=0.95):
-        self.sizes = np.array([s[0].shape[0] for s in systems], np.uint64)
-        pm = np.ascontiguousarray(np.concatenate([s[0] for s in systems]))
-        vm = np.ascontiguousarray(np.concatenate([s[1] for s in systems]))
-        e2 = np.full(len(systems), eps2, np.float32)
-        self.total = pm.shape[0]
-        self.h = lib().nbody_hip_ensemble_create(len(systems), self.sizes.ctypes.data, pm.ctypes.data, vm.ctypes.data,
-                                                 e2.ctypes.data, dt, damping)
-        assert self.h, _native.last_error()
-
-    def run(self, steps, mode):
-        ms = ctypes.c_float()
-        _native.check(lib().nbody_hip_ensemble_run(self.h, steps, mode, ctypes.byref(ms)), "ensemble run")
-        return ms.value
-
-    def read(self):
-        pm = np.zeros((self.total, 4), np.float32)
-        vm = np.zeros((self.total, 4), np.float32)
-        f = np.zeros((self.total, 4), np.float32)
-        _native.check(lib().nbody_hip_ensemble_read(self.h, pm.ctypes.data, vm.ctypes.data, f.ctypes.data), "read")
-        return pm, vm, f
-
-    def split(self, a):
-        return np.split(a, np.cumsum(self.sizes.astype(np.int64))[:-1])
-
-    def close(self):
-        lib().nbody_hip_ensemble_destroy(self.h)
 
 
 @pytest.mark.parametrize("opt", [{}, {"variant": _native.LDS}, {"variant": _native.SGPR, "i_per_lane": 2}])
