@@ -126,6 +126,53 @@ __device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
 
 #define NBODY_ROR1(r) "v_mov_b32_dpp %[" #r "], %[" #r "] wave_ror:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
 
+// The first pair of a step whose reactions are summed apart (the batched
+// kernel's steps, below): it starts the step's reaction partial with v_mul
+// instead of adding to the running reaction; NBODY_SYM_BLOCK_ADD then adds
+// the partial of the step's 8 pairs to the running reaction (Rx, Ry, Rz).
+#define NBODY_SYM_PAIR_EQ_FIRST(u)                                                                     \
+    "v_sub_f32 %[dx], %[xj], %[x" #u "]\n"                                                             \
+    "v_sub_f32 %[dy], %[yj], %[y" #u "]\n"                                                             \
+    "v_sub_f32 %[dz], %[zj], %[z" #u "]\n"                                                             \
+    "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                                           \
+    "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                                  \
+    "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                                  \
+    "v_rsq_f32 %[s], %[s]\n"                                                                           \
+    "s_nop 0\n"                                                                                        \
+    "v_mul_f32 %[t], %[s], %[s]\n"                                                                     \
+    "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
+    "v_mul_f32 %[rx], -%[dx], %[t]\n"                                                                  \
+    "v_mul_f32 %[ry], -%[dy], %[t]\n"                                                                  \
+    "v_mul_f32 %[rz], -%[dz], %[t]\n"                                                                  \
+    "v_fmac_f32 %[ax" #u "], %[dx], %[t]\n"                                                            \
+    "v_fmac_f32 %[ay" #u "], %[dy], %[t]\n"                                                            \
+    "v_fmac_f32 %[az" #u "], %[dz], %[t]\n"
+
+#define NBODY_SYM_PAIR_FIRST(u)                                                                        \
+    "v_sub_f32 %[dx], %[xj], %[x" #u "]\n"                                                             \
+    "v_sub_f32 %[dy], %[yj], %[y" #u "]\n"                                                             \
+    "v_sub_f32 %[dz], %[zj], %[z" #u "]\n"                                                             \
+    "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                                           \
+    "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                                  \
+    "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                                  \
+    "v_rsq_f32 %[s], %[s]\n"                                                                           \
+    "s_nop 0\n"                                                                                        \
+    "v_mul_f32 %[t], %[s], %[s]\n"                                                                     \
+    "v_mul_f32 %[t], %[t], %[s]\n"                                                                     \
+    "v_mul_f32 %[s], %[mj], %[t]\n"                                                                    \
+    "v_mul_f32 %[t], -%[n" #u "], %[t]\n"                                                               \
+    "v_mul_f32 %[rx], %[dx], %[t]\n"                                                                   \
+    "v_mul_f32 %[ry], %[dy], %[t]\n"                                                                   \
+    "v_mul_f32 %[rz], %[dz], %[t]\n"                                                                   \
+    "v_fmac_f32 %[ax" #u "], %[dx], %[s]\n"                                                            \
+    "v_fmac_f32 %[ay" #u "], %[dy], %[s]\n"                                                            \
+    "v_fmac_f32 %[az" #u "], %[dz], %[s]\n"
+
+#define NBODY_SYM_BLOCK_ADD                                                                            \
+    "v_add_f32 %[Rx], %[Rx], %[rx]\n"                                                                  \
+    "v_add_f32 %[Ry], %[Ry], %[ry]\n"                                                                  \
+    "v_add_f32 %[Rz], %[Rz], %[rz]\n"
+
 // The chain's five temporaries (d, then s, then t = s^3) are either left to
 // the compiler or, with PIN, held in v21, v25, v29, v33 and v34. On gfx950
 // the issue rate of one instruction stream depends on which registers hold
@@ -134,18 +181,28 @@ __device__ __forceinline__ void upper_pair(int t, int& P, int& Q)
 // own allocation measures at the top of that range. In the batched kernel
 // the compiler picks a slow one, and pinning raises that kernel's rate by
 // about 7 %.
+// The batched kernel's steps (PIN) also sum each step's 8 reaction terms
+// apart and add that partial to the running reaction (3 more VALU per 8
+// pairs, 1.2 % of its rate): the running j-side sums then take 64 roundings
+// per tile instead of 512, which keeps every system of the 512-system
+// ensemble within 1e-5 of the FP64 sum on its evolved states (1.8e-5
+// without). The single-system kernel keeps one running sum: it holds the
+// bound on its configured state without the extra adds.
 template <bool PIN>
 __device__ __forceinline__ void sym_step8(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
                                           Lanes<8>& acc, const Lanes<8>& me, const float (&mi)[8], float eps2)
 {
     float dx, dy, dz, s, t;
     if constexpr (PIN) {
-        asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
+        float qx, qy, qz;
+        asm volatile(NBODY_SYM_PAIR_FIRST(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
                      NBODY_SYM_PAIR(4) NBODY_SYM_PAIR(5) NBODY_SYM_PAIR(6) NBODY_SYM_PAIR(7)
+                     NBODY_SYM_BLOCK_ADD
                      NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj)
-                     NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
-                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry),
-                       [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                     NBODY_ROR1(Rx) NBODY_ROR1(Ry) NBODY_ROR1(Rz)
+                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [Rx] "+v"(rx), [Ry] "+v"(ry), [Rz] "+v"(rz),
+                       [rx] "=&v"(qx), [ry] "=&v"(qy), [rz] "=&v"(qz),
+                       [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
                        [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
                        [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
                        [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
@@ -260,12 +317,15 @@ __device__ __forceinline__ void sym_step8_eq(float& xj, float& yj, float& zj, fl
 {
     float dx, dy, dz, s, t;
     if constexpr (PIN) {
-        asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3)
+        float qx, qy, qz;
+        asm volatile(NBODY_SYM_PAIR_EQ_FIRST(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3)
                      NBODY_SYM_PAIR_EQ(4) NBODY_SYM_PAIR_EQ(5) NBODY_SYM_PAIR_EQ(6) NBODY_SYM_PAIR_EQ(7)
+                     NBODY_SYM_BLOCK_ADD
                      NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj)
-                     NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
-                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [rx] "+v"(rx), [ry] "+v"(ry),
-                       [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
+                     NBODY_ROR1(Rx) NBODY_ROR1(Ry) NBODY_ROR1(Rz)
+                     : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [Rx] "+v"(rx), [Ry] "+v"(ry), [Rz] "+v"(rz),
+                       [rx] "=&v"(qx), [ry] "=&v"(qy), [rz] "=&v"(qz),
+                       [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
                        [ax3] "+v"(acc.x[3]), [ax4] "+v"(acc.x[4]), [ax5] "+v"(acc.x[5]), [ax6] "+v"(acc.x[6]),
                        [ax7] "+v"(acc.x[7]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
                        [ay3] "+v"(acc.y[3]), [ay4] "+v"(acc.y[4]), [ay5] "+v"(acc.y[5]), [ay6] "+v"(acc.y[6]),
@@ -529,6 +589,9 @@ __device__ __forceinline__ void sym_diag_step8(float& xj, float& yj, float& zj, 
 #undef NBODY_ROR_EQ
 #undef NBODY_SYM_PAIR_M
 #undef NBODY_SYM_PAIR_EQ_M
+#undef NBODY_SYM_PAIR_EQ_FIRST
+#undef NBODY_SYM_PAIR_FIRST
+#undef NBODY_SYM_BLOCK_ADD
 
 #undef NBODY_SYM_PAIR_EQ
 #undef NBODY_SYM_PAIR
