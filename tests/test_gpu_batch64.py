@@ -193,3 +193,30 @@ def test_options_the_fp64_batches_refuse():
                                         DAMPING, None)
     assert h, _native.last_error()
     L.nbody_hip_ensemble_destroy(h)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("compensated", [-1, 2])
+def test_batched_size_boundaries_unequal_masses(oracle, compensated):
+    """Every size boundary of the batched layouts, with the reference's own
+    ICs (randomUnitBodies: masses in U(0.5, 1), so the general pair weights):
+    block edges of both block sizes, and the thresholds above which a system
+    runs as one-sided rows (2^17 with FP64 accumulation, 2^18 without)."""
+    sizes = [1, 2, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1023, 1024, 1025,
+             2 ** 17 - 1, 2 ** 17, 2 ** 17 + 1, 2 ** 18 - 1, 2 ** 18, 2 ** 18 + 1]
+    systems = [ics.random_unit_bodies(n, seed=20 + k) for k, n in enumerate(sizes)]
+    h, sz = create(systems, compensated=compensated)
+    _, _, f = run(h, sz, 1, _native.BATCHED_GRAPH)
+    assert np.array_equal(f[0][:, :3], np.zeros((1, 3), np.float32))
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for k, (pm, _) in enumerate(systems[1:], start=1):
+        n = pm.shape[0]
+        # both ends of the system (first and last blocks) and random bodies
+        idx = np.unique(np.concatenate([np.arange(min(n, 8)), np.arange(max(0, n - 8), n),
+                                        rng.choice(n, min(n, 48), replace=False)]))
+        f64 = oracle.fp64_forces(pm, EPS2, idx)
+        rel = np.linalg.norm(f[k][idx, :3].astype(np.float64) - f64, axis=1) / np.linalg.norm(f64, axis=1)
+        worst = max(worst, float(rel.max()))
+        assert rel.max() <= TOL_MAX, (n, compensated, float(rel.max()))
+    print(f"compensated={compensated}: worst {worst:.3e} over {len(sizes) - 1} systems")
