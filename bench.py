@@ -37,7 +37,8 @@ in one run, src/nbody.cu:1024-1053):
   cpu_baseline  the reference's CPU_SOA_tiled (src/nbody_CPU_SOA_tiled.cpp:51-115)
                 compiled from its sources, else the oracle's restatement, on
                 this host's physical cores, at N = 98304 and 16384, plus one
-                CPU step of every ensemble system and of sampled requests.
+                CPU step of every ensemble system and of sampled requests, and
+                the N = 2^20 Plummer force extrapolated from 2^18 of its bodies.
 
 Single workloads on their own (--workload): ensemble, served, plummer, share
 (one GPU's share of the N = 98304 system split over --shares GPUs: its i
@@ -185,7 +186,8 @@ def _cpu_rate(impl, integ, pm, vm, eps2, dt, damping, budget_s, max_steps=20):
     return float(np.median(times)), len(times), algo, pm
 
 
-def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s, topology, ensemble_sizes=None, served_reqs=None):
+def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s, topology, ensemble_sizes=None, served_reqs=None,
+                 plummer=False):
     """The reference's CPU path on this host (rank 0, one GPU only): the
     reference library compiled from its sources when present (kind
     "reference"), else the oracle's restatement (kind "port"). `topology` is
@@ -255,6 +257,18 @@ def cpu_baseline(pm_host, vm_host, eps2, dt, damping, budget_s, topology, ensemb
         out["ensemble"] = {"value": round(work / t_force / 1e9, 3), "ms_per_step": round(t_force * 1e3, 2),
                            "sample": f"one step of each of the {len(ensemble_sizes)} systems, one after another "
                                      "(CPU_SOA, CPU_SOA_tiled where N % 4096 == 0)"}
+    if plummer:
+        # the N = 2^20 Plummer sphere extrapolated from one force on its first
+        # 2^18 bodies (a quarter of the sphere's sample, 1/16 of the work):
+        # the rate holds at this size, and the full force would take ~12 s
+        p, _ = ics.plummer(1 << 20, seed=1)
+        sub = np.ascontiguousarray(p[: 1 << 18])
+        ms = impl.cpu_soa_tiled(sub, np.float32(1e-4))[1]
+        rate = float(1 << 18) ** 2 / (ms / 1e3)
+        out["plummer"] = {"value": round(rate / 1e9, 3),
+                          "ms_per_force_extrapolated": round(float(1 << 20) ** 2 / rate * 1e3, 1),
+                          "sample": "one CPU_SOA_tiled force on the first 2^18 bodies of the N = 2^20 Plummer sphere "
+                                    "(eps^2 = 1e-4), extrapolated to N^2 = 2^40 interactions"}
     if served_reqs is not None:
         # one CPU step of every 20th request of the served trace
         t_force, work, picked, total = 0.0, 0.0, 0, 0.0
@@ -992,10 +1006,11 @@ def main():
         ens_sizes, _ = ensemble_systems(args.systems)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pm_h, vm_h, np.float32(0.01), 0.016, 0.95, args.cpu_budget_s, topology,
-                                           ensemble_sizes=ens_sizes, served_reqs=reqs)
+                                           ensemble_sizes=ens_sizes, served_reqs=reqs, plummer=ens_sizes is not None)
         if ens_sizes is not None:
             out["ensemble"]["vs_cpu"] = round(out["ensemble"]["value"] / out["cpu_baseline"]["ensemble"]["value"], 1)
             out["served"]["vs_cpu"] = round(out["served"]["value"] / out["cpu_baseline"]["served"]["value"], 1)
+            out["plummer"]["vs_cpu"] = round(out["plummer"]["value"] / out["cpu_baseline"]["plummer"]["value"], 1)
     print(json.dumps(out))
 
 
