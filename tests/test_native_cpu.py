@@ -183,3 +183,58 @@ def test_size_and_range_limits_fail_before_any_device_call():
     assert L.nbody_hip_force(fake, fake, 0.01, 100, 90, 11, ctypes.byref(o), None) != 0
     assert "i range" in _native.last_error()
     assert L.nbody_hip_force(fake, fake, 0.01, 100, 101, 0, ctypes.byref(o), None) != 0
+
+
+C_CLIENT = r"""
+#include "nbody_hip.h"
+#include "nbody_ics.h"
+#include "nbody_engine.h"
+#include "nbody_serve.h"
+#include <stdio.h>
+#include <stdlib.h>
+
+/* A C caller of the library: the calls below touch no GPU. */
+int main(void)
+{
+    size_t n = 1000;
+    float *pm = (float *)malloc(16 * n), *vm = (float *)malloc(16 * n);
+    nbody_hip_options opt = {0};
+    opt.compensated = 2;
+    if (nbody_hip_abi_version() != NBODY_HIP_ABI_VERSION) return 2;
+    if (nbody_uniform_ball(pm, vm, n, 1u, 1.0f, 1.0f, 1.0f) != 0) return 3;
+    printf("%zu %zu %.9g\n", nbody_hip_request_bytes(n), nbody_hip_request_bytes_opt(n, &opt), pm[0]);
+    free(pm);
+    free(vm);
+    return 0;
+}
+"""
+
+
+def test_headers_are_plain_c_and_link_from_c(tmp_path):
+    """include/*.h compile as C99 (pedantic) and C++11, and a C program built
+    against them links libnbody_hip.so and runs calls that need no GPU."""
+    import shutil
+    import subprocess
+    gcc, gxx = shutil.which("gcc"), shutil.which("g++")
+    if not gcc or not gxx:
+        pytest.skip("gcc/g++ not on PATH")
+    inc = os.path.join(ROOT, "include")
+    libdir = os.path.dirname(_native.LIB_PATH)
+    src = tmp_path / "client.c"
+    src.write_text(C_CLIENT)
+    for cmd in ([gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", inc, "-fsyntax-only", "-x", "c",
+                 str(src)],
+                [gxx, "-std=c++11", "-Wall", "-Werror", "-I", inc, "-fsyntax-only", "-x", "c++", str(src)]):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+    exe = tmp_path / "client"
+    r = subprocess.run([gcc, "-std=c99", "-I", inc, str(src), "-o", str(exe), "-L", libdir, "-lnbody_hip",
+                        "-Wl,-rpath," + libdir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    fp32_bytes, fp64_bytes, x0 = r.stdout.split()
+    from nbody_amd import serve
+    assert int(fp32_bytes) == serve.request_bytes(1000) and int(fp64_bytes) == serve.request_bytes(1000, 2)
+    pm, _ = ics.uniform_ball(1000, seed=1, radius=1.0)
+    assert np.float32(x0) == pm[0, 0]
