@@ -44,7 +44,7 @@ extern "C" {
 enum {
     NBODY_HIP_AOS = 0,       /* one i per lane, whole j range (src/nbody_GPU_AOS.cu:42-94) */
     NBODY_HIP_LDS = 1,       /* j tiles in LDS (src/nbody_GPU_Shared.cu:42-103) */
-    NBODY_HIP_DPP = 2,       /* wave64 DPP row broadcast (src/nbody_GPU_Shuffle.cu:44-107) */
+    NBODY_HIP_DPP = 2,       /* j tile passed around the wave64 by DPP (src/nbody_GPU_Shuffle.cu:44-107) */
     NBODY_HIP_SGPR = 3,      /* scalar-memory broadcast (src/nbody_GPU_AOS_const.cu:45-121) */
     NBODY_HIP_SYMMETRIC = 4, /* each pair once, no global atomics (src/nbody_GPU_AOS_tiled.cu:44-220) */
     NBODY_HIP_AUTO = -1      /* the autotuned plan, else SYMMETRIC for whole systems of N >= 1024, else LDS */
@@ -60,7 +60,7 @@ enum {
  * tasks divide into the resident waves changes how evenly the launch ends. */
 typedef struct nbody_hip_options {
     int variant;        /* NBODY_HIP_* or NBODY_HIP_AUTO */
-    int i_per_lane;     /* 1, 2, 4 (symmetric 2, 4, 8), or 0 = auto */
+    int i_per_lane;     /* 1, 2, 4 (DPP also 8; symmetric 2, 4, 8), or 0 = auto */
     int jslice;         /* j bodies per slice (symmetric: block size), or 0 = auto */
     int compensated;    /* 1 TwoSum-compensated sums, 0 plain, -1 auto (on for N > 65536);
                            2 every term added in FP64 and the partial sums kept in FP64, rounded
@@ -80,7 +80,7 @@ typedef struct nbody_hip_options {
 float ComputeGravitation_HIP_AOS(float *force, float *posMass, float softeningSquared, size_t N);
 /* LDS-tiled: replaces ComputeGravitation_GPU_Shared (src/nbody_GPU_Shared.cu:83-103). */
 float ComputeGravitation_HIP_LDS(float *force, float *posMass, float softeningSquared, size_t N);
-/* DPP broadcast: replaces ComputeGravitation_GPU_Shuffle (src/nbody_GPU_Shuffle.cu:91-107). */
+/* DPP tile rotation: replaces ComputeGravitation_GPU_Shuffle (src/nbody_GPU_Shuffle.cu:91-107). */
 float ComputeGravitation_HIP_DPP(float *force, float *posMass, float softeningSquared, size_t N);
 /* Scalar-memory broadcast: replaces ComputeGravitation_GPU_AOS_const (src/nbody_GPU_AOS_const.cu:84-121). */
 float ComputeGravitation_HIP_SGPR(float *force, float *posMass, float softeningSquared, size_t N);

@@ -126,6 +126,13 @@ ForcePlan default_plan(int variant, size_t N, size_t n, int cus, const nbody_hip
         return p;
     }
     p.ipl = (opt && opt->i_per_lane > 0) ? opt->i_per_lane : 1;
+    if (variant == VAR_DPP && !(opt && opt->i_per_lane > 0)) {
+        // the rotation's 4 moves per step are shared by the lane's i bodies:
+        // as many as leave enough waves (measured best: 8 at N = 98304 and
+        // 196608, 4 at 32768 and 65536, 2 at 16384, 1 at 8192)
+        const double work = (double)n * (double)N;
+        p.ipl = work >= 6.0e9 ? 8 : work >= 1.0e9 ? 4 : work >= 2.0e8 ? 2 : 1;
+    }
     if (opt && opt->jslice > 0) {
         p.jslice = opt->jslice;
     } else {
@@ -192,8 +199,8 @@ int validate_plan(const ForcePlan& p, size_t N)
         }
         return OK;
     }
-    if (p.ipl != 1 && p.ipl != 2 && p.ipl != 4) {
-        set_error("i_per_lane must be 1, 2 or 4 (got %d)", p.ipl);
+    if (p.ipl != 1 && p.ipl != 2 && p.ipl != 4 && !(p.ipl == 8 && p.variant == VAR_DPP)) {
+        set_error("i_per_lane must be 1, 2 or 4 (8 also for the DPP variant; got %d)", p.ipl);
         return E_ARG;
     }
     return OK;
@@ -724,7 +731,8 @@ float nbody_hip_autotune(float* force, const float* posMass, float softeningSqua
                 }
             }
         } else {
-            for (int ipl : {1, 2, 4}) {
+            for (int ipl : {1, 2, 4, 8}) {
+                if (ipl == 8 && fam != VAR_DPP) continue;
                 o.i_per_lane = ipl;
                 o.jslice = 0;
                 ForcePlan q = default_plan(fam, N, N, d->cus, &o);

@@ -92,36 +92,6 @@ NBODY_BODY_CHAIN(body_body_chain_v, "v")
 NBODY_BODY_CHAIN(body_body_chain_s, "s")
 #undef NBODY_BODY_CHAIN
 
-// The chain with the j body taken from lane K of each 16-lane row (DPP
-// row_newbcast:K on the three differences and on the mass product), for the
-// row-broadcast kernel: the broadcast costs no instruction. The j registers
-// are loaded from memory, never written by a VALU op just before, so the DPP
-// read needs no wait state.
-template <int K>
-__device__ __forceinline__ void body_body_chain_dpp(float xj, float yj, float zj, float mj, float xi, float yi,
-                                                    float zi, float eps2, float& ax, float& ay, float& az)
-{
-    float dx, dy, dz, s, t;
-    asm volatile("v_sub_f32_dpp %[dx], %[xj], %[xi] row_newbcast:%[k] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
-                 "v_sub_f32_dpp %[dy], %[yj], %[yi] row_newbcast:%[k] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
-                 "v_sub_f32_dpp %[dz], %[zj], %[zi] row_newbcast:%[k] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
-                 "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"
-                 "v_fmac_f32 %[s], %[dy], %[dy]\n"
-                 "v_fmac_f32 %[s], %[dz], %[dz]\n"
-                 "v_rsq_f32 %[s], %[s]\n"
-                 "s_nop 0\n"
-                 "v_mul_f32_dpp %[t], %[mj], %[s] row_newbcast:%[k] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
-                 "v_mul_f32 %[s], %[s], %[s]\n"
-                 "v_mul_f32 %[s], %[t], %[s]\n"
-                 "v_fmac_f32 %[ax], %[dx], %[s]\n"
-                 "v_fmac_f32 %[ay], %[dy], %[s]\n"
-                 "v_fmac_f32 %[az], %[dz], %[s]\n"
-                 : [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz),
-                   [s] "=&v"(s), [t] "=&v"(t)
-                 : [xj] "v"(xj), [yj] "v"(yj), [zj] "v"(zj), [mj] "v"(mj), [xi] "v"(xi), [yi] "v"(yi),
-                   [zi] "v"(zi), [eps] "s"(eps2), [k] "i"(K));
-}
-
 // body_body for the kernels: the assembly chain when no Newton step is asked
 // for, with the j body in VGPRs (SGPR_J false) or SGPRs (SGPR_J true).
 template <bool NEWTON, bool SGPR_J>
@@ -131,16 +101,6 @@ __device__ __forceinline__ void interact(float xj, float yj, float zj, float mj,
     if constexpr (NEWTON) body_body<true>(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
     else if constexpr (SGPR_J) body_body_chain_s(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
     else body_body_chain_v(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
-}
-
-// DPP row_newbcast:K (gfx90a and later): every lane of each 16-lane row reads
-// lane K of its row. The combine pass folds it into the consuming VOP2
-// (v_sub_f32_dpp / v_mul_f32_dpp), so the broadcast costs no extra VALU op.
-template <int K>
-__device__ __forceinline__ float row_bcast(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                 0x150 + K, 0xf, 0xf, false));
 }
 
 // Per-lane state of I i bodies (or their accumulators).

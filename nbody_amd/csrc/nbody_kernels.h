@@ -12,7 +12,7 @@ namespace nbody {
 enum Variant : int {
     VAR_AOS = 0,        // one i per lane over the whole j range (the ladder's first rung)
     VAR_LDS = 1,        // 256-body j tiles staged in LDS, broadcast ds_read_b128
-    VAR_DPP = 2,        // 16 j per row in VGPRs, DPP row_newbcast into the VALU op
+    VAR_DPP = 2,        // 64-body j tile in VGPRs, rotated one lane per step (DPP wave_ror:1)
     VAR_SGPR = 3,       // scalar loads: j lives in SGPRs, used as VALU operands
     VAR_SYMMETRIC = 4,  // each pair once, reaction kept in LDS (no global atomics)
     VAR_COUNT = 5

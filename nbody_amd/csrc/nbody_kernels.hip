@@ -200,37 +200,41 @@ __global__ __launch_bounds__(kBlock) void k_force_lds64(ForceParams p)
     }
 }
 
-template <int I, bool NEWTON, int K>
-__device__ __forceinline__ void dpp_steps(const float4& b, float eps2, const Lanes<I>& me, Lanes<I>& acc)
-{
-    if constexpr (K < 16) {
-#pragma unroll
-        for (int u = 0; u < I; ++u) {
-            if constexpr (NEWTON)
-                body_body<true>(row_bcast<K>(b.x), row_bcast<K>(b.y), row_bcast<K>(b.z), row_bcast<K>(b.w),
-                                me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u], acc.z[u]);
-            else
-                body_body_chain_dpp<K>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
-                                       acc.z[u]);
-        }
-        dpp_steps<I, NEWTON, K + 1>(b, eps2, me, acc);
-    }
-}
-
-// Each 16-lane row loads the same 16 j bodies (one float4 per lane); step K
-// reads lane K of the row through DPP row_newbcast folded into the VALU op.
-// The wave64 counterpart of the reference's warp-shuffle broadcast
-// (src/nbody_GPU_Shuffle.cu:44-89). Bodies past the last full group of 16 go
-// through the scalar-load sweep.
+// The wave64 counterpart of the reference's warp-shuffle kernel
+// (src/nbody_GPU_Shuffle.cu:44-89), where each lane loads one j body and the
+// warp broadcasts every lane's body in turn: here each lane loads one body of
+// a 64-body tile, every lane meets the body it holds with its I i bodies, and
+// the tile then moves one lane along the wave (DPP wave_ror:1, one full-rate
+// v_mov_b32_dpp per component), so after 64 steps every lane has met all 64.
+// Per interaction that is the 12 VALU + 1 v_rsq_f32 chain plus 4/I moves;
+// the row broadcast folded into the chain's operands (row_newbcast on three
+// subtractions and a product, four DPP operands per interaction) measured
+// 0.40 of the FP32 peak at N = 98304 against 0.46 for this. The lane holding
+// body i loads tile body (i mod 64) first (shift = i0 mod 64 for the first
+// body i0 of the range) and then the ones below it, wrapping around: a body's
+// terms come in an order fixed by its own index, so an i-range evaluation
+// (base, n) gives each body the bits of the whole-system one. Compensated
+// plans fold the terms per 128 as the other variants do. Bodies past the
+// last full tile go through the scalar-load sweep.
 template <int I, bool NEWTON>
-__device__ __forceinline__ void sweep_dpp(const float4* __restrict__ src, int len, float eps2,
+__device__ __forceinline__ void sweep_dpp(const float4* __restrict__ src, int len, int shift, float eps2,
                                           const Lanes<I>& me, Lanes<I>& acc)
 {
-    const int lane16 = threadIdx.x & 15;
+    const int first = (int)(threadIdx.x + shift) & 63;
     int g = 0;
-    for (; g + 16 <= len; g += 16) {
-        const float4 b = src[g + lane16];
-        dpp_steps<I, NEWTON, 0>(b, eps2, me, acc);
+    for (; g + 64 <= len; g += 64) {
+        const float4 b = src[g + first];
+        float xj = b.x, yj = b.y, zj = b.z, mj = b.w;
+        for (int k = 0; k < 64; ++k) {
+#pragma unroll
+            for (int u = 0; u < I; ++u)
+                interact<NEWTON, false>(xj, yj, zj, mj, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
+                                        acc.z[u]);
+            xj = wave_ror1(xj);
+            yj = wave_ror1(yj);
+            zj = wave_ror1(zj);
+            mj = wave_ror1(mj);
+        }
     }
     if (g < len) sweep_sgpr<I, NEWTON>(src + g, len - g, eps2, me, acc);
 }
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_force_dpp(ForceParams p)
     const int len = min(p.n - j0, p.jslice);
     const float4* __restrict__ src = p.pm + j0;
     if constexpr (!COMP) {
-        sweep_dpp<I, NEWTON>(src, len, p.eps2, me, hi);
+        sweep_dpp<I, NEWTON>(src, len, p.i0 & 63, p.eps2, me, hi);
         store<I>(p, hi, nullptr);
     } else {
         Lanes<I> lo;
@@ -253,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_force_dpp(ForceParams p)
         for (int c = 0; c < len; c += kChunk) {
             Lanes<I> part;
             zero<I>(part);
-            sweep_dpp<I, NEWTON>(src + c, min(kChunk, len - c), p.eps2, me, part);
+            sweep_dpp<I, NEWTON>(src + c, min(kChunk, len - c), p.i0 & 63, p.eps2, me, part);
             fold<I>(hi, lo, part);
         }
         store<I>(p, hi, &lo);
@@ -602,17 +606,20 @@ __global__ void k_energy_fp64_final(const double* __restrict__ partial, int bloc
     acc[c] = s + e;
 }
 
-#define NBODY_KERNEL_FAMILY(NAME, KERNEL)                                                        \
+// MAX_I: the most i bodies per lane a family is built for (8 only where the
+// j body costs an instruction per step to reach the lanes: the DPP rotation).
+#define NBODY_KERNEL_FAMILY(NAME, KERNEL, MAX_I)                                                 \
     struct NAME {                                                                                \
+        static constexpr int kMaxI = MAX_I;                                                      \
         template <int I, bool C, bool N>                                                         \
         static void go(dim3 g, hipStream_t s, const ForceParams& p)                              \
         {                                                                                        \
             KERNEL<I, C, N><<<g, kBlock, 0, s>>>(p);                                             \
         }                                                                                        \
     };
-NBODY_KERNEL_FAMILY(FamSgpr, k_force_sgpr)
-NBODY_KERNEL_FAMILY(FamLds, k_force_lds)
-NBODY_KERNEL_FAMILY(FamDpp, k_force_dpp)
+NBODY_KERNEL_FAMILY(FamSgpr, k_force_sgpr, 4)
+NBODY_KERNEL_FAMILY(FamLds, k_force_lds, 4)
+NBODY_KERNEL_FAMILY(FamDpp, k_force_dpp, 8)
 #undef NBODY_KERNEL_FAMILY
 
 template <class Fam, int I>
@@ -634,6 +641,12 @@ hipError_t go_family(const ForcePlan& plan, dim3 g, hipStream_t s, const ForcePa
     case 1: go_i<Fam, 1>(plan, g, s, p); break;
     case 2: go_i<Fam, 2>(plan, g, s, p); break;
     case 4: go_i<Fam, 4>(plan, g, s, p); break;
+    case 8:
+        if constexpr (Fam::kMaxI >= 8) {
+            go_i<Fam, 8>(plan, g, s, p);
+            break;
+        }
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -98,6 +98,31 @@ def test_plan_options(oracle, variant, ipl, comp, newton):
         assert np.all(f[:, 3] == -7.0)
 
 
+@pytest.mark.parametrize("comp", [0, 1])
+@pytest.mark.parametrize("newton", [0, 1])
+def test_dpp_eight_per_lane(oracle, comp, newton):
+    """The DPP tile rotation with 8 i bodies per lane (its default shape at
+    large N), on slices that end in a partial tile (1000, 5003) and on whole
+    tiles (256); the variants whose j body costs no move per step are built
+    for at most 4 and refuse 8."""
+    pm, _ = bodies(5003, seed=2)
+    d = dev(pm)
+    f64 = oracle.fp64_forces(pm, EPS2)
+    for jslice in (256, 1000, 5003):
+        f = run_force(d, EPS2, variant=_native.DPP, i_per_lane=8, jslice=jslice, compensated=comp, newton=newton)
+        mx, rms = force_errors(f, f64)
+        tmax, trms = tolerances(jslice, comp)
+        assert mx <= tmax and rms <= trms, (jslice, comp, newton, mx, rms)
+        assert np.all(f[:, 3] == -7.0)
+    plan = _native.Options()
+    for variant in (_native.LDS, _native.SGPR):
+        o = _native.Options.make(variant=variant, i_per_lane=8)
+        assert _native.lib().nbody_hip_plan(5003, 5003, ctypes.byref(o), ctypes.byref(plan)) != 0
+    o = _native.Options.make(variant=_native.DPP)
+    _native.check(_native.lib().nbody_hip_plan(98304, 98304, ctypes.byref(o), ctypes.byref(plan)), "plan")
+    assert plan.i_per_lane == 8
+
+
 @pytest.mark.parametrize("shape", SYM_SHAPES)
 @pytest.mark.parametrize("newton", [0, 1])
 def test_symmetric_shapes(oracle, shape, newton):
