@@ -362,43 +362,11 @@ __device__ __forceinline__ void sym_step8_eq(float& xj, float& yj, float& zj, fl
     }
 }
 
-// The same steps for 4 and 2 bodies per lane (the smaller block shapes).
-__device__ __forceinline__ void sym_step4(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
-                                          Lanes<4>& acc, const Lanes<4>& me, const float (&mi)[4], float eps2)
-{
-    float dx, dy, dz, s, t;
-    asm volatile(NBODY_SYM_PAIR(0) NBODY_SYM_PAIR(1) NBODY_SYM_PAIR(2) NBODY_SYM_PAIR(3)
-                 NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj) NBODY_ROR1(mj)
-                 NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
-                 : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [mj] "+v"(mj), [rx] "+v"(rx), [ry] "+v"(ry),
-                   [rz] "+v"(rz), [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]),
-                   [ax3] "+v"(acc.x[3]), [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]),
-                   [ay3] "+v"(acc.y[3]), [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]),
-                   [az3] "+v"(acc.z[3]), [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s),
-                   [t] "=&v"(t)
-                 : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]), [y0] "v"(me.y[0]),
-                   [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]), [z0] "v"(me.z[0]), [z1] "v"(me.z[1]),
-                   [z2] "v"(me.z[2]), [z3] "v"(me.z[3]), [n0] "v"(mi[0]), [n1] "v"(mi[1]), [n2] "v"(mi[2]),
-                   [n3] "v"(mi[3]), [eps] "s"(eps2));
-}
-
-__device__ __forceinline__ void sym_step4_eq(float& xj, float& yj, float& zj, float& rx, float& ry, float& rz,
-                                          Lanes<4>& acc, const Lanes<4>& me, float eps2)
-{
-    float dx, dy, dz, s, t;
-    asm volatile(NBODY_SYM_PAIR_EQ(0) NBODY_SYM_PAIR_EQ(1) NBODY_SYM_PAIR_EQ(2) NBODY_SYM_PAIR_EQ(3)
-                 NBODY_ROR1(xj) NBODY_ROR1(yj) NBODY_ROR1(zj)
-                 NBODY_ROR1(rx) NBODY_ROR1(ry) NBODY_ROR1(rz)
-                 : [xj] "+v"(xj), [yj] "+v"(yj), [zj] "+v"(zj), [rx] "+v"(rx), [ry] "+v"(ry), [rz] "+v"(rz),
-                   [ax0] "+v"(acc.x[0]), [ax1] "+v"(acc.x[1]), [ax2] "+v"(acc.x[2]), [ax3] "+v"(acc.x[3]),
-                   [ay0] "+v"(acc.y[0]), [ay1] "+v"(acc.y[1]), [ay2] "+v"(acc.y[2]), [ay3] "+v"(acc.y[3]),
-                   [az0] "+v"(acc.z[0]), [az1] "+v"(acc.z[1]), [az2] "+v"(acc.z[2]), [az3] "+v"(acc.z[3]),
-                   [dx] "=&v"(dx), [dy] "=&v"(dy), [dz] "=&v"(dz), [s] "=&v"(s), [t] "=&v"(t)
-                 : [x0] "v"(me.x[0]), [x1] "v"(me.x[1]), [x2] "v"(me.x[2]), [x3] "v"(me.x[3]), [y0] "v"(me.y[0]),
-                   [y1] "v"(me.y[1]), [y2] "v"(me.y[2]), [y3] "v"(me.y[3]), [z0] "v"(me.z[0]), [z1] "v"(me.z[1]),
-                   [z2] "v"(me.z[2]), [z3] "v"(me.z[3]), [eps] "s"(eps2));
-}
-
+// The same step for 2 bodies per lane (the smallest block shapes). With 4
+// bodies per lane the compiler's schedule of the generic loop measured
+// faster than this chain order (blocks of 256 at N = 24576..65536: 0.54-0.71
+// of the FP32 peak against 0.53-0.68; the same at N = 16384), so that shape
+// runs the generic loop; with 2 and 8 the chain order is faster.
 __device__ __forceinline__ void sym_step2(float& xj, float& yj, float& zj, float& mj, float& rx, float& ry, float& rz,
                                           Lanes<2>& acc, const Lanes<2>& me, const float (&mi)[2], float eps2)
 {
@@ -625,11 +593,6 @@ __device__ __forceinline__ void sym_phases(const float4* __restrict__ pm, int n,
                 if constexpr (A == 8 && !NEWTON) {
                     if constexpr (EQ) sym_step8_eq<PIN>(xj, yj, zj, rx, ry, rz, acc, me, eps2);
                     else sym_step8<PIN>(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2);
-                    continue;
-                }
-                if constexpr (A == 4 && !NEWTON) {
-                    if constexpr (EQ) sym_step4_eq(xj, yj, zj, rx, ry, rz, acc, me, eps2);
-                    else sym_step4(xj, yj, zj, mj, rx, ry, rz, acc, me, mi, eps2);
                     continue;
                 }
                 if constexpr (A == 2 && !NEWTON) {
