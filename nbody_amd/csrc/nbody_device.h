@@ -92,13 +92,50 @@ NBODY_BODY_CHAIN(body_body_chain_v, "v")
 NBODY_BODY_CHAIN(body_body_chain_s, "s")
 #undef NBODY_BODY_CHAIN
 
+// The chain without the mass, for j ranges whose bodies all have one mass m0
+// (the caller multiplies the sums by m0 once): inv^3 = inv * (inv * inv),
+// the product the chain above forms for m = 1, so with m0 = 1 (or any power
+// of two) the sums have the same bits. 11 VALU + 1 v_rsq_f32. The mass is an
+// operand the chain does not read: with it in use, the j record stays one
+// 16-byte load (gfx950 has no 12-byte scalar load, and the scalar-broadcast
+// kernel would otherwise get vector loads for x, y, z).
+#define NBODY_BODY_CHAIN_EQ(NAME, JC)                                                                  \
+    __device__ __forceinline__ void NAME(float xj, float yj, float zj, float mj, float xi, float yi,   \
+                                         float zi, float eps2, float& ax, float& ay, float& az)        \
+    {                                                                                                  \
+        float dx, dy, dz, s, t;                                                                        \
+        asm volatile("v_sub_f32 %[dx], %[xj], %[xi]\n"                                                 \
+                     "v_sub_f32 %[dy], %[yj], %[yi]\n"                                                 \
+                     "v_sub_f32 %[dz], %[zj], %[zi]\n"                                                 \
+                     "v_fma_f32 %[s], %[dx], %[dx], %[eps]\n"                                          \
+                     "v_fmac_f32 %[s], %[dy], %[dy]\n"                                                 \
+                     "v_fmac_f32 %[s], %[dz], %[dz]\n"                                                 \
+                     "v_rsq_f32 %[s], %[s]\n"                                                          \
+                     "s_nop 0\n"                                                                       \
+                     "v_mul_f32 %[t], %[s], %[s]\n"                                                    \
+                     "v_mul_f32 %[s], %[s], %[t]\n"                                                    \
+                     "v_fmac_f32 %[ax], %[dx], %[s]\n"                                                 \
+                     "v_fmac_f32 %[ay], %[dy], %[s]\n"                                                 \
+                     "v_fmac_f32 %[az], %[dz], %[s]\n"                                                 \
+                     : [ax] "+v"(ax), [ay] "+v"(ay), [az] "+v"(az), [dx] "=&v"(dx), [dy] "=&v"(dy),     \
+                       [dz] "=&v"(dz), [s] "=&v"(s), [t] "=&v"(t)                                      \
+                     : [xj] JC(xj), [yj] JC(yj), [zj] JC(zj), [mj] JC(mj), [xi] "v"(xi), [yi] "v"(yi),  \
+                       [zi] "v"(zi), [eps] "s"(eps2));                                                 \
+    }
+NBODY_BODY_CHAIN_EQ(body_body_chain_eq_v, "v")
+NBODY_BODY_CHAIN_EQ(body_body_chain_eq_s, "s")
+#undef NBODY_BODY_CHAIN_EQ
+
 // body_body for the kernels: the assembly chain when no Newton step is asked
-// for, with the j body in VGPRs (SGPR_J false) or SGPRs (SGPR_J true).
-template <bool NEWTON, bool SGPR_J>
+// for, with the j body in VGPRs (SGPR_J false) or SGPRs (SGPR_J true). EQ
+// leaves the mass out (the mass-free chain; with a Newton step, m = 1).
+template <bool NEWTON, bool SGPR_J, bool EQ = false>
 __device__ __forceinline__ void interact(float xj, float yj, float zj, float mj, float xi, float yi, float zi,
                                          float eps2, float& ax, float& ay, float& az)
 {
-    if constexpr (NEWTON) body_body<true>(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
+    if constexpr (NEWTON) body_body<true>(xj, yj, zj, EQ ? 1.0f : mj, xi, yi, zi, eps2, ax, ay, az);
+    else if constexpr (EQ && SGPR_J) body_body_chain_eq_s(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
+    else if constexpr (EQ) body_body_chain_eq_v(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
     else if constexpr (SGPR_J) body_body_chain_s(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
     else body_body_chain_v(xj, yj, zj, mj, xi, yi, zi, eps2, ax, ay, az);
 }
@@ -139,7 +176,7 @@ __device__ __forceinline__ void fold(Lanes<I>& hi, Lanes<I>& lo, const Lanes<I>&
 // One sweep over j bodies in groups of G: the G loads are issued before the
 // G x I interaction chains that consume them (the chains are assembly the
 // compiler neither reorders nor unrolls around), then the remainder.
-template <int I, bool NEWTON, bool SGPR_J, int G, class Load>
+template <int I, bool NEWTON, bool SGPR_J, int G, bool EQ = false, class Load>
 __device__ __forceinline__ void sweep_grouped(const Load& load, int len, float eps2, const Lanes<I>& me,
                                               Lanes<I>& acc)
 {
@@ -152,26 +189,26 @@ __device__ __forceinline__ void sweep_grouped(const Load& load, int len, float e
         for (int q = 0; q < G; ++q)
 #pragma unroll
             for (int u = 0; u < I; ++u)
-                interact<NEWTON, SGPR_J>(b[q].x, b[q].y, b[q].z, b[q].w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u],
-                                         acc.y[u], acc.z[u]);
+                interact<NEWTON, SGPR_J, EQ>(b[q].x, b[q].y, b[q].z, b[q].w, me.x[u], me.y[u], me.z[u], eps2,
+                                             acc.x[u], acc.y[u], acc.z[u]);
     }
     for (; j < len; ++j) {
         const float4 b = load(j);
 #pragma unroll
         for (int u = 0; u < I; ++u)
-            interact<NEWTON, SGPR_J>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
-                                     acc.z[u]);
+            interact<NEWTON, SGPR_J, EQ>(b.x, b.y, b.z, b.w, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
+                                         acc.z[u]);
     }
 }
 
 // j bodies read with wave-uniform addresses: the compiler emits s_load_dwordx16
 // (four bodies per load) and the VALU ops take x_j, y_j, z_j, m_j as SGPR
 // operands, so no instruction is spent on the broadcast.
-template <int I, bool NEWTON>
+template <int I, bool NEWTON, bool EQ = false>
 __device__ __forceinline__ void sweep_sgpr(const float4* __restrict__ src, int len, float eps2,
                                            const Lanes<I>& me, Lanes<I>& acc)
 {
-    sweep_grouped<I, NEWTON, true, 8>([src](int k) { return src[k]; }, len, eps2, me, acc);
+    sweep_grouped<I, NEWTON, true, 8, EQ>([src](int k) { return src[k]; }, len, eps2, me, acc);
 }
 
 // Read-only view of body records for kernels whose body pointers are not
@@ -180,12 +217,12 @@ __device__ __forceinline__ void sweep_sgpr(const float4* __restrict__ src, int l
 // wave-uniform reads stay scalar loads (s_load) into SGPR operands.
 typedef const __attribute__((address_space(4))) float const_body_float;
 
-template <int I, bool NEWTON>
+template <int I, bool NEWTON, bool EQ = false>
 __device__ __forceinline__ void sweep_const(const float4* src4, int len, float eps2, const Lanes<I>& me,
                                             Lanes<I>& acc)
 {
     const const_body_float* src = (const const_body_float*)(const void*)src4;
-    sweep_grouped<I, NEWTON, true, 8>(
+    sweep_grouped<I, NEWTON, true, 8, EQ>(
         [src](int k) { return make_float4(src[4 * k + 0], src[4 * k + 1], src[4 * k + 2], src[4 * k + 3]); }, len,
         eps2, me, acc);
 }

@@ -71,29 +71,88 @@ __device__ __forceinline__ void store(const ForceParams& p, const Lanes<I>& hi, 
     }
 }
 
-template <int I, bool COMP, bool NEWTON>
-__global__ __launch_bounds__(kBlock) void k_force_sgpr(ForceParams p)
+// True (on every lane) when every body of the slice src[0, len) has the mass
+// of its first, m0 (uniform initial conditions: unit masses, m = 1/N). The
+// one-sided kernels then run the mass-free chain (11 VALU + 1 v_rsq_f32 per
+// interaction instead of 12) and multiply the slice's sums by m0 once; for
+// m0 a power of two that gives the same bits as the general chain, otherwise
+// it differs by the rounding of that one product. Each wave votes on its own
+// (a ballot, no barrier): a barrier would count as a possible store to the
+// bodies, and the SGPR kernel's wave-uniform j loads after it would no longer
+// be scalar loads.
+__device__ __forceinline__ bool uniform_mass(const float4* __restrict__ src, int len, float& m0)
 {
-    Lanes<I> me, hi;
-    load_i<I>(p, me);
-    zero<I>(hi);
-    const int j0 = blockIdx.y * p.jslice;
-    const int len = min(p.n - j0, p.jslice);
-    const float4* __restrict__ src = p.pm + j0;
+    m0 = src[0].w;
+    bool ok = true;
+    for (int k = threadIdx.x & 63; k < len; k += 64) ok &= src[k].w == m0;
+    return __builtin_amdgcn_ballot_w64(!ok) == 0;
+}
+
+template <int I>
+__device__ __forceinline__ void scale(Lanes<I>& a, float m)
+{
+#pragma unroll
+    for (int u = 0; u < I; ++u) {
+        a.x[u] *= m;
+        a.y[u] *= m;
+        a.z[u] *= m;
+    }
+}
+
+// A slice's sums into hi through sweep(c, len, acc), which adds the terms of
+// bodies [c, c + len) of the slice: one plain sum, or (COMP) plain sums of
+// kChunk terms folded into the TwoSum pair (hi, lo).
+template <int I, bool COMP, class Sweep>
+__device__ __forceinline__ void slice_sums(const Sweep& sweep, int len, Lanes<I>& hi, Lanes<I>& lo)
+{
     if constexpr (!COMP) {
-        sweep_sgpr<I, NEWTON>(src, len, p.eps2, me, hi);
-        store<I>(p, hi, nullptr);
+        sweep(0, len, hi);
     } else {
-        Lanes<I> lo;
-        zero<I>(lo);
         for (int c = 0; c < len; c += kChunk) {
             Lanes<I> part;
             zero<I>(part);
-            sweep_sgpr<I, NEWTON>(src + c, min(kChunk, len - c), p.eps2, me, part);
+            sweep(c, min(kChunk, len - c), part);
             fold<I>(hi, lo, part);
         }
-        store<I>(p, hi, &lo);
     }
+}
+
+// slice_sums with the mass-free chain when the slice's masses are all m0
+// (then the sums are multiplied by m0), else the general chain; SWEEP<EQ>
+// is the kernel's sweep. The branch is taken once per slice, outside the
+// loops, so each path keeps its own straight loop.
+template <int I, bool COMP, class SweepEq, class SweepGen>
+__device__ __forceinline__ void slice_sums_mass(const float4* __restrict__ src, int len, const SweepEq& eq_sweep,
+                                                const SweepGen& gen_sweep, Lanes<I>& hi, Lanes<I>& lo)
+{
+    float m0;
+    if (uniform_mass(src, len, m0)) {
+        slice_sums<I, COMP>(eq_sweep, len, hi, lo);
+        scale<I>(hi, m0);
+        if constexpr (COMP) scale<I>(lo, m0);
+    } else {
+        slice_sums<I, COMP>(gen_sweep, len, hi, lo);
+    }
+}
+
+template <int I, bool COMP, bool NEWTON>
+__global__ __launch_bounds__(kBlock) void k_force_sgpr(ForceParams p)
+{
+    Lanes<I> me, hi, lo;
+    load_i<I>(p, me);
+    zero<I>(hi);
+    zero<I>(lo);
+    const int j0 = blockIdx.y * p.jslice;
+    const int len = min(p.n - j0, p.jslice);
+    const float4* __restrict__ src = p.pm + j0;
+    const float eps2 = p.eps2;
+    // j bodies through the constant address space: with the mass vote's
+    // vector loads of the slice in the kernel, the compiler keeps only one of
+    // the two sweeps' wave-uniform loads scalar otherwise
+    slice_sums_mass<I, COMP>(
+        src, len, [&](int c, int l, Lanes<I>& a) { sweep_const<I, NEWTON, true>(src + c, l, eps2, me, a); },
+        [&](int c, int l, Lanes<I>& a) { sweep_const<I, NEWTON>(src + c, l, eps2, me, a); }, hi, lo);
+    store<I>(p, hi, COMP ? &lo : nullptr);
 }
 
 // The reference's first rung (src/nbody_GPU_AOS.cu:42-75): one i body per
@@ -126,6 +185,8 @@ __global__ __launch_bounds__(kBlock) void k_force_lds(ForceParams p)
     const int j0 = blockIdx.y * p.jslice;
     const int len = min(p.n - j0, p.jslice);
     const float4* __restrict__ src = p.pm + j0;
+    float m0;
+    const bool eq = uniform_mass(src, len, m0);
     int buf = 0;
     for (int t = 0; t < len; t += kBlock, buf ^= 1) {
         const int k = t + threadIdx.x;
@@ -136,8 +197,13 @@ __global__ __launch_bounds__(kBlock) void k_force_lds(ForceParams p)
         zero<I>(part);
         Lanes<I>& acc = COMP ? part : hi;
         const float4* t4 = tile[buf];
-        sweep_grouped<I, NEWTON, false, 4>([t4](int q) { return t4[q]; }, cnt, p.eps2, me, acc);
+        if (eq) sweep_grouped<I, NEWTON, false, 4, true>([t4](int q) { return t4[q]; }, cnt, p.eps2, me, acc);
+        else sweep_grouped<I, NEWTON, false, 4>([t4](int q) { return t4[q]; }, cnt, p.eps2, me, acc);
         if constexpr (COMP) fold<I>(hi, lo, part);
+    }
+    if (eq) {
+        scale<I>(hi, m0);
+        if constexpr (COMP) scale<I>(lo, m0);
     }
     store<I>(p, hi, COMP ? &lo : nullptr);
 }
@@ -216,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_force_lds64(ForceParams p)
 // (base, n) gives each body the bits of the whole-system one. Compensated
 // plans fold the terms per 128 as the other variants do. Bodies past the
 // last full tile go through the scalar-load sweep.
-template <int I, bool NEWTON>
+template <int I, bool NEWTON, bool EQ>
 __device__ __forceinline__ void sweep_dpp(const float4* __restrict__ src, int len, int shift, float eps2,
                                           const Lanes<I>& me, Lanes<I>& acc)
 {
@@ -228,40 +294,33 @@ __device__ __forceinline__ void sweep_dpp(const float4* __restrict__ src, int le
         for (int k = 0; k < 64; ++k) {
 #pragma unroll
             for (int u = 0; u < I; ++u)
-                interact<NEWTON, false>(xj, yj, zj, mj, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
-                                        acc.z[u]);
+                interact<NEWTON, false, EQ>(xj, yj, zj, mj, me.x[u], me.y[u], me.z[u], eps2, acc.x[u], acc.y[u],
+                                            acc.z[u]);
             xj = wave_ror1(xj);
             yj = wave_ror1(yj);
             zj = wave_ror1(zj);
-            mj = wave_ror1(mj);
+            if constexpr (!EQ) mj = wave_ror1(mj);
         }
     }
-    if (g < len) sweep_sgpr<I, NEWTON>(src + g, len - g, eps2, me, acc);
+    if (g < len) sweep_sgpr<I, NEWTON, EQ>(src + g, len - g, eps2, me, acc);
 }
 
 template <int I, bool COMP, bool NEWTON>
 __global__ __launch_bounds__(kBlock) void k_force_dpp(ForceParams p)
 {
-    Lanes<I> me, hi;
+    Lanes<I> me, hi, lo;
     load_i<I>(p, me);
     zero<I>(hi);
+    zero<I>(lo);
     const int j0 = blockIdx.y * p.jslice;
     const int len = min(p.n - j0, p.jslice);
     const float4* __restrict__ src = p.pm + j0;
-    if constexpr (!COMP) {
-        sweep_dpp<I, NEWTON>(src, len, p.i0 & 63, p.eps2, me, hi);
-        store<I>(p, hi, nullptr);
-    } else {
-        Lanes<I> lo;
-        zero<I>(lo);
-        for (int c = 0; c < len; c += kChunk) {
-            Lanes<I> part;
-            zero<I>(part);
-            sweep_dpp<I, NEWTON>(src + c, min(kChunk, len - c), p.i0 & 63, p.eps2, me, part);
-            fold<I>(hi, lo, part);
-        }
-        store<I>(p, hi, &lo);
-    }
+    const float eps2 = p.eps2;
+    const int shift = p.i0 & 63;
+    slice_sums_mass<I, COMP>(
+        src, len, [&](int c, int l, Lanes<I>& a) { sweep_dpp<I, NEWTON, true>(src + c, l, shift, eps2, me, a); },
+        [&](int c, int l, Lanes<I>& a) { sweep_dpp<I, NEWTON, false>(src + c, l, shift, eps2, me, a); }, hi, lo);
+    store<I>(p, hi, COMP ? &lo : nullptr);
 }
 
 // ---- symmetric: each pair once, reaction kept on chip ----------------------

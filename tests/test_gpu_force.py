@@ -123,6 +123,37 @@ def test_dpp_eight_per_lane(oracle, comp, newton):
     assert plan.i_per_lane == 8
 
 
+@pytest.mark.parametrize("variant", [_native.SGPR, _native.LDS, _native.DPP])
+@pytest.mark.parametrize("comp", [0, 1])
+@pytest.mark.parametrize("newton", [0, 1])
+def test_one_sided_equal_mass(oracle, variant, comp, newton):
+    """A j slice whose bodies share one mass takes the mass-free chain and
+    multiplies its sums by that mass once. With unit masses that gives the
+    bits of the general chain: the same system with one zero-mass body
+    appended to the slice (which sends the slice down the general chain and
+    adds exact zeros) must give the first n bodies the same forces. A mass of
+    1/4 scales them exactly, and both stay within the oracle's bounds."""
+    n = 4096
+    pm, _ = ics.uniform_ball(n, seed=3, radius=ics.ball_radius(n))
+    assert np.all(pm[:, 3] == 1.0)
+    extra = np.concatenate([pm, np.array([[3.0, -2.0, 1.0, 0.0]], np.float32)])
+    quarter = pm.copy()
+    quarter[:, 3] = 0.25
+    f64 = oracle.fp64_forces(pm, EPS2)
+    d, d_extra, d_quarter = dev(pm), dev(extra), dev(quarter)
+    for ipl in ((1, 2, 4, 8) if variant == _native.DPP else (1, 2, 4)):
+        opt = dict(variant=variant, i_per_lane=ipl, jslice=8192, compensated=comp, newton=newton)
+        a = run_force(d, EPS2, **opt)
+        b = run_force(d_extra, EPS2, **opt)
+        c = run_force(d_quarter, EPS2, **opt)
+        assert np.array_equal(a[:, :3].view(np.uint32), b[:n, :3].view(np.uint32)), (ipl, "unit vs general")
+        assert np.array_equal((a[:, :3] * np.float32(0.25)).view(np.uint32), c[:, :3].view(np.uint32)), (ipl, "1/4")
+        mx, rms = force_errors(a, f64)
+        tmax, trms = tolerances(n, comp)
+        assert mx <= tmax and rms <= trms, (ipl, mx, rms)
+        assert np.all(a[:, 3] == -7.0)
+
+
 @pytest.mark.parametrize("shape", SYM_SHAPES)
 @pytest.mark.parametrize("newton", [0, 1])
 def test_symmetric_shapes(oracle, shape, newton):
