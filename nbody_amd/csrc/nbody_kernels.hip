@@ -302,25 +302,72 @@ __device__ __forceinline__ void sweep_dpp(const float4* __restrict__ src, int le
             if constexpr (!EQ) mj = wave_ror1(mj);
         }
     }
-    if (g < len) sweep_sgpr<I, NEWTON, EQ>(src + g, len - g, eps2, me, acc);
+    // (through the constant address space: scalar loads, and no vector
+    // registers for the group of j bodies in flight)
+    if (g < len) sweep_const<I, NEWTON, EQ>(src + g, len - g, eps2, me, acc);
 }
 
+// Compensated plans keep the TwoSum residues of the DPP kernel in LDS, one
+// column per thread (no thread reads another's, so no barrier), written once
+// per kChunk terms: with 8 bodies per lane the residues in VGPRs took the
+// kernel to 144 VGPRs, 3 waves per SIMD. Same arithmetic, same bits.
 template <int I, bool COMP, bool NEWTON>
 __global__ __launch_bounds__(kBlock) void k_force_dpp(ForceParams p)
 {
-    Lanes<I> me, hi, lo;
+    Lanes<I> me, hi;
     load_i<I>(p, me);
     zero<I>(hi);
-    zero<I>(lo);
     const int j0 = blockIdx.y * p.jslice;
     const int len = min(p.n - j0, p.jslice);
     const float4* __restrict__ src = p.pm + j0;
     const float eps2 = p.eps2;
     const int shift = p.i0 & 63;
-    slice_sums_mass<I, COMP>(
-        src, len, [&](int c, int l, Lanes<I>& a) { sweep_dpp<I, NEWTON, true>(src + c, l, shift, eps2, me, a); },
-        [&](int c, int l, Lanes<I>& a) { sweep_dpp<I, NEWTON, false>(src + c, l, shift, eps2, me, a); }, hi, lo);
-    store<I>(p, hi, COMP ? &lo : nullptr);
+    auto eq_sweep = [&](int c, int l, Lanes<I>& a) { sweep_dpp<I, NEWTON, true>(src + c, l, shift, eps2, me, a); };
+    auto gen_sweep = [&](int c, int l, Lanes<I>& a) { sweep_dpp<I, NEWTON, false>(src + c, l, shift, eps2, me, a); };
+    if constexpr (!COMP) {
+        Lanes<I> lo;
+        zero<I>(lo);
+        slice_sums_mass<I, false>(src, len, eq_sweep, gen_sweep, hi, lo);
+        store<I>(p, hi, nullptr);
+    } else {
+        __shared__ float res[3 * I][kBlock];
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int r = 0; r < 3 * I; ++r) res[r][t] = 0.f;
+        auto sums = [&](const auto& sweep) {
+            for (int c = 0; c < len; c += kChunk) {
+                Lanes<I> part;
+                zero<I>(part);
+                sweep(c, min(kChunk, len - c), part);
+#pragma unroll
+                for (int u = 0; u < I; ++u) {
+                    float e;
+                    hi.x[u] = two_sum(hi.x[u], part.x[u], e);
+                    res[u][t] += e;
+                    hi.y[u] = two_sum(hi.y[u], part.y[u], e);
+                    res[I + u][t] += e;
+                    hi.z[u] = two_sum(hi.z[u], part.z[u], e);
+                    res[2 * I + u][t] += e;
+                }
+            }
+        };
+        float m0;
+        const bool eq = uniform_mass(src, len, m0);
+        if (eq) sums(eq_sweep);
+        else sums(gen_sweep);
+        Lanes<I> lo;
+#pragma unroll
+        for (int u = 0; u < I; ++u) {
+            lo.x[u] = res[u][t];
+            lo.y[u] = res[I + u][t];
+            lo.z[u] = res[2 * I + u][t];
+        }
+        if (eq) {
+            scale<I>(hi, m0);
+            scale<I>(lo, m0);
+        }
+        store<I>(p, hi, &lo);
+    }
 }
 
 // ---- symmetric: each pair once, reaction kept on chip ----------------------
