@@ -5,8 +5,9 @@
 //     m = b*256*I + u*256 + t (u < I), so each wave's i loads and stores are
 //     contiguous float4 runs;
 //   * blockIdx.y selects a j slice [y*jslice, min(n, (y+1)*jslice));
-//   * the variants differ only in how the wave-uniform j body reaches the
-//     lanes (scalar loads into SGPRs, LDS broadcast, DPP row broadcast);
+//   * the variants differ only in how the j bodies reach the lanes: one
+//     wave-uniform body at a time (scalar loads into SGPRs, an LDS
+//     broadcast), or a 64-body tile that DPP moves one lane per step;
 //   * slice partials (float4, plus a float4 of TwoSum residues when
 //     compensated) are summed by k_reduce in slice order, so a result depends
 //     on the plan only, never on scheduling.
