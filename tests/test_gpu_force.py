@@ -356,15 +356,23 @@ def test_deterministic(variant):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [_native.SGPR, _native.LDS, _native.DPP])
-def test_i_range_equals_full(variant):
+@pytest.mark.parametrize("variant,ipl", [(_native.SGPR, 2), (_native.LDS, 2), (_native.DPP, 2), (_native.DPP, 4),
+                                         (_native.DPP, 8)])
+@pytest.mark.parametrize("masses", ["unequal", "one"])
+def test_i_range_equals_full(variant, ipl, masses):
     """The base/n entry (the multi-GPU i-slice of src/nbody_multiGPU_shared.cuh)
-    gives each body exactly the force of the full evaluation under one plan."""
-    pm, _ = bodies(9000, seed=5)
+    gives each body exactly the force of the full evaluation under one plan,
+    with the reference's unequal masses and with one mass (the mass-free
+    chain); the DPP tile rotation orders each body's terms by its own index,
+    whatever the range's first body."""
+    if masses == "one":
+        pm, _ = ics.uniform_ball(9000, seed=5)
+    else:
+        pm, _ = bodies(9000, seed=5)
     d = dev(pm)
-    opt = dict(variant=variant, i_per_lane=2, jslice=1024, compensated=1)
+    opt = dict(variant=variant, i_per_lane=ipl, jslice=1024, compensated=1)
     full = run_force(d, EPS2, **opt)
-    for base, count in [(0, 1), (4500, 1125), (8999, 1), (1, 8999), (2048, 2048)]:
+    for base, count in [(0, 1), (4500, 1125), (8999, 1), (1, 8999), (2048, 2048), (37, 5000)]:
         part = run_force(d, EPS2, base=base, count=count, **opt)
         assert np.array_equal(part.view(np.uint32), full[base:base + count].view(np.uint32)), (base, count)
 
