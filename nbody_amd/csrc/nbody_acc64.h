@@ -41,30 +41,19 @@ __device__ __forceinline__ double wave_ror1_f64(double v)
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
-// Block pair (P, Q), P < Q, on one wave with A bodies of block P per lane
-// (B = 64 A): the rotation of nbody_sym.h (the j tile's position, mass and
-// FP64 reaction move one lane per step), each term accumulated in FP64. Every
-// tile is visited by this wave alone, so its reactions go straight from the
-// lanes to their slots. Tiles past the end of a ragged last block are
-// skipped: their bodies are padding, whose slots no reduction reads.
-template <int A, bool NEWTON>
-__device__ __forceinline__ void sym_pair64(const float4* __restrict__ pm, int n, float eps2, const SlotMap64& slots,
-                                           int P, int Q)
+// The tiles of block pair (P, Q) against this lane's A bodies of P (see
+// sym_pair64). With EQ every body of both blocks has mass m0: the term
+// weights leave it out (inv^3 on the i side, -inv^3 on the j side: one
+// conversion to FP64 instead of three, no mass products, the masses do not
+// rotate) and the j-side partials are multiplied by m0 at the store; for
+// m0 = 1 the terms are the general path's bits.
+template <int A, bool NEWTON, bool EQ>
+__device__ __forceinline__ void sym_tiles64(const float4* __restrict__ pm, int n, float eps2, const SlotMap64& slots,
+                                            int P, int Q, const Lanes<A>& me, const float (&mi)[A], double m0,
+                                            Lanes64<A>& acc)
 {
     constexpr int B = 64 * A;
     const int lane = threadIdx.x;
-    Lanes<A> me;
-    float mi[A];
-#pragma unroll
-    for (int u = 0; u < A; ++u) {
-        const float4 q = pm[P * B + u * 64 + lane];
-        me.x[u] = q.x;
-        me.y[u] = q.y;
-        me.z[u] = q.z;
-        mi[u] = q.w;
-    }
-    Lanes64<A> acc;
-    zero<A>(acc);
     for (int tile = 0; tile < A; ++tile) {
         const int jbase = Q * B + tile * 64;  // wave-uniform
         if (jbase >= n) break;
@@ -82,36 +71,86 @@ __device__ __forceinline__ void sym_pair64(const float4* __restrict__ pm, int n,
                 d2 = __builtin_fmaf(dz, dz, d2);
                 const float inv = rsqrt_f32<NEWTON>(d2);
                 const float inv3 = (inv * inv) * inv;
-                const double si = (double)(mj * inv3);
-                const double sj = (double)(-mi[u] * inv3);
                 const double DX = dx, DY = dy, DZ = dz;
-                acc.x[u] = __builtin_fma(DX, si, acc.x[u]);
-                acc.y[u] = __builtin_fma(DY, si, acc.y[u]);
-                acc.z[u] = __builtin_fma(DZ, si, acc.z[u]);
-                rx = __builtin_fma(DX, sj, rx);
-                ry = __builtin_fma(DY, sj, ry);
-                rz = __builtin_fma(DZ, sj, rz);
+                if constexpr (EQ) {
+                    const double w = (double)inv3;
+                    acc.x[u] = __builtin_fma(DX, w, acc.x[u]);
+                    acc.y[u] = __builtin_fma(DY, w, acc.y[u]);
+                    acc.z[u] = __builtin_fma(DZ, w, acc.z[u]);
+                    rx = __builtin_fma(DX, -w, rx);
+                    ry = __builtin_fma(DY, -w, ry);
+                    rz = __builtin_fma(DZ, -w, rz);
+                } else {
+                    const double si = (double)(mj * inv3);
+                    const double sj = (double)(-mi[u] * inv3);
+                    acc.x[u] = __builtin_fma(DX, si, acc.x[u]);
+                    acc.y[u] = __builtin_fma(DY, si, acc.y[u]);
+                    acc.z[u] = __builtin_fma(DZ, si, acc.z[u]);
+                    rx = __builtin_fma(DX, sj, rx);
+                    ry = __builtin_fma(DY, sj, ry);
+                    rz = __builtin_fma(DZ, sj, rz);
+                }
             }
             xj = wave_ror1(xj);
             yj = wave_ror1(yj);
             zj = wave_ror1(zj);
-            mj = wave_ror1(mj);
+            if constexpr (!EQ) mj = wave_ror1(mj);
             rx = wave_ror1_f64(rx);
             ry = wave_ror1_f64(ry);
             rz = wave_ror1_f64(rz);
         }
         double* __restrict__ jdst = slots.j_side(P, Q, B) + tile * 64 + lane;
-        jdst[0] = rx;
-        jdst[slots.bplane] = ry;
-        jdst[2 * slots.bplane] = rz;
+        jdst[0] = EQ ? rx * m0 : rx;
+        jdst[slots.bplane] = EQ ? ry * m0 : ry;
+        jdst[2 * slots.bplane] = EQ ? rz * m0 : rz;
     }
+}
+
+// Block pair (P, Q), P < Q, on one wave with A bodies of block P per lane
+// (B = 64 A): the rotation of nbody_sym.h (the j tile's position, mass and
+// FP64 reaction move one lane per step), each term accumulated in FP64. Every
+// tile is visited by this wave alone, so its reactions go straight from the
+// lanes to their slots. Tiles past the end of a ragged last block are
+// skipped: their bodies are padding, whose slots no reduction reads. When
+// every body of both blocks has the mass of body P*B (a wave vote; padding
+// has mass 0, so a ragged block never qualifies) the mass-free terms run.
+template <int A, bool NEWTON>
+__device__ __forceinline__ void sym_pair64(const float4* __restrict__ pm, int n, float eps2, const SlotMap64& slots,
+                                           int P, int Q)
+{
+    constexpr int B = 64 * A;
+    const int lane = threadIdx.x;
+    const float m0 = pm[P * B].w;
+    Lanes<A> me;
+    float mi[A];
+    bool same = true;
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+        const float4 q = pm[P * B + u * 64 + lane];
+        me.x[u] = q.x;
+        me.y[u] = q.y;
+        me.z[u] = q.z;
+        mi[u] = q.w;
+        same &= q.w == m0;
+    }
+#pragma unroll
+    for (int u = 0; u < A; ++u) {
+        const int j = Q * B + u * 64 + lane;
+        same &= (j < n) & (pm[min(j, n - 1)].w == m0);
+    }
+    Lanes64<A> acc;
+    zero<A>(acc);
+    const bool eq = __builtin_amdgcn_ballot_w64(!same) == 0;
+    if (eq) sym_tiles64<A, NEWTON, true>(pm, n, eps2, slots, P, Q, me, mi, (double)m0, acc);
+    else sym_tiles64<A, NEWTON, false>(pm, n, eps2, slots, P, Q, me, mi, 1.0, acc);
+    const double scale = eq ? (double)m0 : 1.0;
     double* __restrict__ idst = slots.i_side(P, Q, B) + lane;
     const size_t iplane = slots.i_plane(P);
 #pragma unroll
     for (int u = 0; u < A; ++u) {
-        idst[u * 64] = acc.x[u];
-        idst[iplane + u * 64] = acc.y[u];
-        idst[2 * iplane + u * 64] = acc.z[u];
+        idst[u * 64] = acc.x[u] * scale;
+        idst[iplane + u * 64] = acc.y[u] * scale;
+        idst[2 * iplane + u * 64] = acc.z[u] * scale;
     }
 }
 
