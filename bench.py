@@ -26,6 +26,9 @@ in one run, src/nbody.cu:1024-1053):
                 CPU_AOS with its stated tolerance and a pass flag;
   general_mass  the reference's own ICs (randomUnitBodies, m in U(0.5, 1),
                 src/nbody_util.cpp:137), which take the general pair weights;
+                like the ensemble's timed steps, timed after the same untimed
+                settle (force evaluations on a separate 98304-body ball, so
+                the measured state is not touched);
   ensemble      512 independent systems, batched graph vs one launch per system,
                 and the batched force on the initial and final states of every
                 system against the FP64 direct sum;
@@ -389,7 +392,37 @@ def batched_force_errors(sizes, pm, vm, eps2, compensated=-1):
     return float(d.max()), float(np.sqrt(np.mean(d * d)))
 
 
-def run_ensemble(count, steps, warmup):
+_SETTLE = {}
+
+
+def gpu_settle(ms):
+    """About `ms` milliseconds of untimed force evaluations on a separate
+    98304-body ball (made once), so that the measurement that follows starts
+    at the GPU's sustained speed whatever ran (or idled) before it; the
+    measured workload's own state is not touched. Returns the launches."""
+    if ms <= 0:
+        return 0
+    import torch
+    from nbody_amd import _native, ics
+    L = _native.lib()
+    if not _SETTLE:
+        n = 98304
+        pm_h, _ = ics.uniform_ball(n, seed=11)
+        _SETTLE.update(n=n, pm=torch.from_numpy(pm_h).to("cuda:0"),
+                       f=torch.zeros((n, 4), dtype=torch.float32, device="cuda:0"))
+    n, pm, f = _SETTLE["n"], _SETTLE["pm"], _SETTLE["f"]
+    launches = 0
+    t_end = time.perf_counter() + ms / 1e3
+    while time.perf_counter() < t_end:
+        for _ in range(8):
+            _native.check(L.nbody_hip_force(ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(pm.data_ptr()), 0.01, n,
+                                            0, n, None, None), "settle")
+        torch.cuda.synchronize()
+        launches += 8
+    return launches
+
+
+def run_ensemble(count, steps, warmup, settle_ms=0.0):
     """`count` independent uniform balls, N log-uniform in [256, 16384]
     (seed 4), eps^2 = 0.01, dt = 0.016: batched steps replayed from a graph
     (the systems in two groups, each a persistent force launch and a
@@ -421,6 +454,7 @@ def run_ensemble(count, steps, warmup):
         try:
             ms = ctypes.c_float()
             _native.check(L.nbody_hip_ensemble_run(h, warmup, mode, ctypes.byref(ms)), name)
+            gpu_settle(settle_ms)
             _native.check(L.nbody_hip_ensemble_run(h, steps, mode, ctypes.byref(ms)), name)
             res[name] = {"ms_per_step": round(ms.value / steps, 4),
                          "value": round(inter * steps / (ms.value * 1e-3) / 1e9, 2)}
@@ -445,6 +479,7 @@ def run_ensemble(count, steps, warmup):
         "workload": f"{len(systems)} independent uniform balls, N log-uniform in [256, 16384] (seed 4; "
                     f"{int(sizes.sum())} bodies, sum N^2 = {inter:.4e}), eps^2=0.01, dt=0.016, damping=0.95",
         "modes": res,
+        "settle_ms": settle_ms,
         "speedup_vs_per_system": round(v / res["per_system"]["value"], 2),
         "achieved_tflops": round(20 * v / 1e3, 2),
         "accuracy": {
@@ -601,7 +636,7 @@ def run_plummer(steps):
     }
 
 
-def run_general_mass(n, launches):
+def run_general_mass(n, launches, settle_ms=0.0):
     """The reference's own initial conditions (seedRandom(7) +
     randomUnitBodies(scale 0.9, velscale 8), src/nbody_util.cpp:102-144,
     masses U(0.5, 1)) at the headline N, softening 0.1^2: the default plan's
@@ -623,6 +658,7 @@ def run_general_mass(n, launches):
     for _ in range(3):
         _native.check(L.nbody_hip_force(P(force), P(pm), eps2, n, 0, n, ctypes.byref(opt), None), "force")
     torch.cuda.synchronize()
+    settle = gpu_settle(settle_ms)
     _native.check(L.nbody_hip_kernel_timer(1), "timer")
     t0 = time.perf_counter()
     for _ in range(launches):
@@ -646,6 +682,7 @@ def run_general_mass(n, launches):
         "workload": f"reference ICs randomUnitBodies(seed 7), N={n}, masses U(0.5, 1), eps^2=0.1^2; force "
                     f"evaluations (kernel + slot sum) back to back",
         "kernel_ms": round(kernel_ms, 4),
+        "settle_launches": settle,
         "roofline": {"achieved": round(tf, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(tf / PEAK_FP32_TFLOPS, 4), "kernel": KERNEL_NAMES[plan.variant]},
         "plan": {"variant": _native.VARIANT_NAMES[plan.variant], "i_per_lane": plan.i_per_lane,
@@ -975,7 +1012,7 @@ def main():
         return
 
     if args.workload == "ensemble":
-        body = run_ensemble(args.systems, args.ensemble_steps, 5)
+        body = run_ensemble(args.systems, args.ensemble_steps, 5, args.settle_ms)
         print(json.dumps(standalone(body, "synthetic (seeded uniform balls, unit masses)", "ensemble")))
         return
     if args.workload == "served":
@@ -994,8 +1031,8 @@ def main():
     out, pm_h, vm_h = single_system(args, L, _native, ics, np, torch)
     ens_sizes = reqs = None
     if not args.single_only:
-        out["general_mass"] = run_general_mass(args.n, 20)
-        out["ensemble"] = run_ensemble(args.systems, args.ensemble_steps, 5)
+        out["general_mass"] = run_general_mass(args.n, 20, args.settle_ms)
+        out["ensemble"] = run_ensemble(args.systems, args.ensemble_steps, 5, args.settle_ms)
         out["served"], reqs = run_served(args.requests, args.rate, args.max_active)
         served64, _ = run_served(args.requests, args.rate, args.max_active, compensated=2)
         out["served"]["fp64_accumulation"] = {
