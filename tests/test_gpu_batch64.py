@@ -77,6 +77,38 @@ def test_batched64_equals_fp64_symmetric_plan():
         assert np.array_equal(bits(f[k][:, :3]), bits(single[:, :3])), sizes[k]
 
 
+@pytest.mark.parametrize("shape", [(4, 256), (8, 512)])
+def test_fp64_symmetric_equal_mass_blocks(oracle, shape):
+    """Block pairs whose bodies share one mass take the mass-free FP64 terms
+    (inv^3 converted once, no mass products, the masses do not rotate) and
+    multiply the partials by the mass in FP64 at the store. One mass
+    everywhere (not a power of two), the same with one heavier body (pairs
+    with its block take the general terms), a ragged N (the last block holds
+    zero-mass padding, so its pairs are general) against the oracle; a common
+    mass of 1/4 scales the forces of unit masses exactly. (Unit-mass systems
+    in the batched FP64 mode give the single-system plan's bits:
+    test_batched64_equals_fp64_symmetric_plan.)"""
+    a, b = shape
+    for n in (4 * b, 4 * b + 37):
+        pm, _ = ics.uniform_ball(n, seed=21)
+        pm[:, 3] = 0.37
+        mixed = pm.copy()
+        mixed[2 * b + 5, 3] = 0.6
+        for case, x in (("one mass", pm), ("one heavier body", mixed)):
+            f = run_force(dev(x), EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b, compensated=2)
+            f64 = oracle.fp64_forces(x, EPS2)
+            rel = np.linalg.norm(f[:, :3].astype(np.float64) - f64, axis=1) / np.linalg.norm(f64, axis=1)
+            assert rel.max() <= TOL_MAX and np.sqrt(np.mean(rel ** 2)) <= TOL_RMS, (shape, n, case, rel.max())
+            assert np.all(f[:, 3] == -7.0)
+        unit = pm.copy()
+        unit[:, 3] = 1.0
+        quarter = pm.copy()
+        quarter[:, 3] = 0.25
+        f1 = run_force(dev(unit), EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b, compensated=2)
+        fq = run_force(dev(quarter), EPS2, variant=_native.SYMMETRIC, i_per_lane=a, jslice=b, compensated=2)
+        assert np.array_equal(fq[:, :3], np.float32(0.25) * f1[:, :3]), (shape, n)
+
+
 def test_batched64_small_and_row_systems_against_oracle(oracle):
     # fewer bodies than one block, a ragged pair of blocks, and one system
     # above 2^17 bodies, which runs as one-sided FP64 rows
